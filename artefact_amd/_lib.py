@@ -1,0 +1,102 @@
+"""ctypes binding of the C ABI declared in include/artefact_hip.h.
+
+The shared library is built in-tree (``artefact_amd/libartefact_hip.so``).  There
+is no CPU fallback: when the library is missing or cannot be loaded, every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libartefact_hip.so")
+
+ARTEFACT_MODE_TGV = 0
+ARTEFACT_MODE_TV = 1
+
+ERRORS = {
+    -1: "invalid argument",
+    -2: "invalid geometry",
+    -3: "zero entry in a quantisation table",
+    -4: "HIP runtime failure",
+    -5: "plan used out of order",
+}
+
+
+class ArtefactError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        super().__init__(f"{where}: {ERRORS.get(code, 'unknown error')} (code {code})")
+        self.code = code
+
+
+class artefact_channel(ctypes.Structure):
+    _fields_ = [
+        ("rounded_w", ctypes.c_uint32),
+        ("rounded_h", ctypes.c_uint32),
+        ("block_w", ctypes.c_uint32),
+        ("block_h", ctypes.c_uint32),
+        ("h_samp", ctypes.c_uint32),
+        ("v_samp", ctypes.c_uint32),
+        ("coefs", ctypes.POINTER(ctypes.c_int16)),
+        ("quant", ctypes.POINTER(ctypes.c_uint16)),
+    ]
+
+
+class artefact_params(ctypes.Structure):
+    _fields_ = [
+        ("nchannel", ctypes.c_uint32),
+        ("max_w", ctypes.c_uint32),
+        ("max_h", ctypes.c_uint32),
+        ("weight", ctypes.c_float),
+        ("pweight", ctypes.c_float * 3),
+        ("iterations", ctypes.c_uint32),
+        ("mode", ctypes.c_uint32),
+        ("init_override", ctypes.POINTER(ctypes.POINTER(ctypes.c_float))),
+    ]
+
+
+FLOAT_PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
+
+# (name, restype, argtypes) of every exported symbol of include/artefact_hip.h.
+SIGNATURES = [
+    ("artefact_hip_compute", ctypes.c_int, [ctypes.POINTER(artefact_params), ctypes.POINTER(artefact_channel), FLOAT_PP]),
+    ("artefact_hip_plan_create", ctypes.c_int,
+     [ctypes.POINTER(artefact_params), ctypes.POINTER(artefact_channel), ctypes.POINTER(ctypes.c_void_p)]),
+    ("artefact_hip_plan_init", ctypes.c_int, [ctypes.c_void_p]),
+    ("artefact_hip_plan_iterate", ctypes.c_int, [ctypes.c_void_p]),
+    ("artefact_hip_plan_download", ctypes.c_int, [ctypes.c_void_p, FLOAT_PP]),
+    ("artefact_hip_plan_download_cos", ctypes.c_int, [ctypes.c_void_p, FLOAT_PP]),
+    ("artefact_hip_plan_last_norms", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("artefact_hip_plan_first_gradient", ctypes.c_int, [ctypes.c_void_p, FLOAT_PP]),
+    ("artefact_hip_plan_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("artefact_hip_plan_stream", ctypes.c_void_p, [ctypes.c_void_p]),
+    ("artefact_hip_plan_set_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("artefact_hip_plan_destroy", None, [ctypes.c_void_p]),
+    ("artefact_hip_strerror", ctypes.c_char_p, [ctypes.c_int]),
+]
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads the HIP library once; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, restype, argtypes in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def check(code: int, where: str) -> None:
+    if code != 0:
+        raise ArtefactError(code, where)

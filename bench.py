@@ -1,0 +1,149 @@
+"""Benchmark of the reconstruction loop on one MI355X.
+
+Workload: plain TGV, YCbCr 4:2:0, 2048x1536 (Y 2048x1536, Cb/Cr 1024x768), IJG
+tables at quality 75, 100 iterations, f32, synthetic seeded image.  One step is
+one solve from resident coefficients: the start image (dequantise + IDCT +
+upsample) and the 100-iteration loop (a hipGraph of 200 kernels), inputs already
+in HBM.  ``value`` is Mpixel*iterations/s over the timed steps.
+
+The JSON line also carries
+* ``roofline``: the iteration's algorithmic HBM bytes B = sum_c (20 N + 16 M_c)
+  (x, x_prev, x_new as f32 over the full grid; int16 coefficients and the f32
+  coefficient estimate read and written per component coefficient) divided by
+  the measured mean duration of one iteration's two kernels (HIP events around
+  each launch on the plan's stream), against 8 TB/s;
+* ``cpu_baseline``: the CPU oracle (a single-threaded restatement of the
+  reference's scalar loop) timed on this host on a bounded sample of the same
+  workload (same geometry, 30 iterations, loop only).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(coefs, W, H):
+    N = W * H
+    return sum(20 * N + 16 * c.rounded_px_count for c in coefs)
+
+
+def cpu_baseline(wl, iters):
+    import oracle
+
+    W, H = wl["max_w"], wl["max_h"]
+    _, secs = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, return_time=True)
+    return {
+        "value": W * H * iters / secs / 1e6,
+        "unit": "Mpixel*it/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{W}x{H} 4:2:0 TGV, {iters} iterations, loop only ({secs:.2f} s), single-threaded C restatement "
+                  f"of pipeline_scalar (oracle/oracle.c, -O2 -ffp-contract=off)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="tgv_420_2048")
+    ap.add_argument("--cpu-iters", type=int, default=30)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU form is a design note only)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    import artefact_amd
+    from artefact_amd import synth
+
+    wl = synth.workload(args.workload, seed=0)
+    W, H, iters = wl["max_w"], wl["max_h"], wl["iterations"]
+    coefs = wl["coefs"]
+    C = len(coefs)
+    print(f"bench: {args.workload} {W}x{H} C={C} iters={iters} steps={args.steps} warmup={args.warmup}",
+          file=sys.stderr, flush=True)
+
+    plan = artefact_amd.Plan(coefs, wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"], graph=True)
+    for _ in range(args.warmup):
+        plan.init()
+        plan.iterate()
+    plan.download()  # synchronises
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.init()
+        plan.iterate()
+    plan.last_norms()  # synchronises the plan's stream
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = W * H * iters * args.steps / elapsed / 1e6
+
+    # Per-kernel durations with HIP events on the plan's stream.
+    plan.init()
+    prof = plan.profile()
+    t_iter_ms = prof["k_gradient_ms"] + prof["k_project_ms"]
+    B = algorithmic_bytes(coefs, W, H)
+    achieved = B / (t_iter_ms * 1e-3) / 1e9
+    N = W * H
+    impl_bytes = sum(28 * N + 12 * c.rounded_px_count for c in coefs)
+
+    out = plan.download()
+    finite = all(bool(np.isfinite(p).all()) for p in out)
+    plan.close()
+
+    res = {
+        "metric": "Mpixel*iterations/s (plain TGV 4:2:0 2048x1536, 100 iterations)",
+        "value": round(value, 2),
+        "unit": "Mpixel*it/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, artefact_amd/synth.py)",
+        "config": {
+            "workload": "plain TGV YCbCr 4:2:0 2048x1536, IJG q75, 100 iterations, weight 0.3, pweight 0.001",
+            "width": W, "height": H, "subsampling": "4:2:0", "iterations": iters, "quality": wl["quality"],
+            "time_to_100_iter_objective_ms": round(ms_per_step, 4),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "one iteration = k_gradient + k_project",
+            "algorithmic_bytes_per_iteration": B,
+            "implementation_bytes_per_iteration": impl_bytes,
+            "k_gradient_ms": round(prof["k_gradient_ms"], 5),
+            "k_project_ms": round(prof["k_project_ms"], 5),
+        },
+        "finite_output": finite,
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(wl, args.cpu_iters)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

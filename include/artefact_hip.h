@@ -1,0 +1,112 @@
+/*
+ * artefact_hip -- MI355X (gfx950) reconstruction loop of artefact-lib behind a C ABI.
+ *
+ * The entry points replace the crate-private compute function of
+ * backend/artefact-lib/pipeline_scalar/mod.rs:17-26 (identical signatures at
+ * pipeline_simd_8/mod.rs:22-31 and pipeline_simd_adaptive/mod.rs:19-28), which
+ * backend/artefact-lib/lib.rs:122-153 calls in joint (nchannel = 3) or
+ * separate (nchannel = 1, one call per component) mode:
+ *
+ *   fn compute(nchannel: usize, coefs: Vec<Coefficient>, weight: f32, pweight: [f32; 3],
+ *              iterations: usize, max_rounded_px_w: u32, max_rounded_px_h: u32,
+ *              max_rounded_px_count: usize) -> Vec<Vec<f32>>
+ *
+ * Plain pointers and sizes only.  Every call is re-entrant: a plan owns its own
+ * HIP stream and device buffers, and the library holds no global mutable state,
+ * so concurrent calls from several host threads (the reference's separate-
+ * components mode fans out over rayon workers) are safe.
+ */
+#ifndef ARTEFACT_HIP_H
+#define ARTEFACT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. */
+#define ARTEFACT_OK              0
+#define ARTEFACT_E_ARGUMENT     -1  /* null pointer, nchannel not in 1..3, bad mode */
+#define ARTEFACT_E_GEOMETRY     -2  /* plane sizes not multiples of 8, inconsistent block counts, samp factor not 1/2, component larger than the grid */
+#define ARTEFACT_E_QUANT        -3  /* a quantisation table entry is 0 (the reference divides by q^2, compute_step_prob.rs:29) */
+#define ARTEFACT_E_HIP          -4  /* HIP runtime failure (allocation, launch, copy) */
+#define ARTEFACT_E_STATE        -5  /* plan used out of order (iterate before init) */
+
+#define ARTEFACT_MODE_TGV 0  /* TV + second-order term with strength weight (compute_step.rs:42-48) */
+#define ARTEFACT_MODE_TV  1  /* TV only: the same loop with weight forced to 0 */
+
+/* One colour component: backend/artefact-lib/jpeg/mod.rs:8-27 (Coefficient).
+ * The reference carries coefficients and the quantisation table as f32 holding
+ * i16 / u16 values (backend/artefact-lib/jpeg/zune.rs:33-45); they pass through
+ * this boundary as the integers themselves, bit-exact. */
+typedef struct {
+    uint32_t rounded_w, rounded_h;  /* rounded_px_w / rounded_px_h: multiples of 8 */
+    uint32_t block_w, block_h;      /* rounded_w / 8, rounded_h / 8 */
+    uint32_t h_samp, v_samp;        /* horizontal/vertical_samp_factor: upsampling onto the max grid, 1 or 2 */
+    const int16_t *coefs;           /* dct_coefs: block_w*block_h*64, raster block order, natural order in a block */
+    const uint16_t *quant;          /* quant_table: 64, natural order */
+} artefact_channel;
+
+/* The scalar arguments of compute().  pweight is indexed by channel as in
+ * compute_step.rs:26,30 (so a separate-mode call with nchannel = 1 uses pweight[0]). */
+typedef struct {
+    uint32_t nchannel;              /* 1..3 */
+    uint32_t max_w, max_h;          /* max_rounded_px_w / max_rounded_px_h (lib.rs:108-116) */
+    float weight;                   /* second-order weight (ignored in ARTEFACT_MODE_TV) */
+    float pweight[3];               /* coefficient-distance weights */
+    uint32_t iterations;
+    uint32_t mode;                  /* ARTEFACT_MODE_TGV or ARTEFACT_MODE_TV */
+    const float *const *init_override; /* NULL, or nchannel host planes of max_w*max_h replacing the decoded start image */
+} artefact_params;
+
+/* One-shot drop-in for compute(): copies the coefficients to the GPU, runs the
+ * init and `iterations` iterations, and writes nchannel planes of max_w*max_h f32
+ * into caller-allocated host buffers out_planes[c].  Channel 0 stays level-shifted
+ * (the caller adds 128, lib.rs:160-162), chroma stays centred at 0.
+ * Replaces pipeline_scalar/mod.rs:17-78. */
+int artefact_hip_compute(const artefact_params *params, const artefact_channel *channels,
+                         float *const *out_planes);
+
+/* Resident form of the same call, for callers that keep the image in HBM across
+ * calls (and for timing the iteration loop alone). */
+typedef struct artefact_hip_plan artefact_hip_plan;
+
+/* Validates the geometry, allocates device buffers on the current device and
+ * copies coefficients and quantisation tables (the Coefficient conversion of
+ * pipeline_scalar/mod.rs:27-30). */
+int artefact_hip_plan_create(const artefact_params *params, const artefact_channel *channels,
+                             artefact_hip_plan **out_plan);
+/* Start image: ScalarCoef::from + Aux::init (pipeline_simd_8/coef.rs:88-118 order,
+ * utils/aux.rs:45-65), or params->init_override when it was given. */
+int artefact_hip_plan_init(artefact_hip_plan *plan);
+/* The iteration loop, pipeline_scalar/mod.rs:45-75.  Asynchronous on the plan's stream. */
+int artefact_hip_plan_iterate(artefact_hip_plan *plan);
+/* Synchronises the plan's stream and copies the planes out (pipeline_scalar/mod.rs:77). */
+int artefact_hip_plan_download(artefact_hip_plan *plan, float *const *out_planes);
+/* Copies the current coefficient estimate `cos` (block order, M_c floats per
+ * channel), the last projection's clamped DCT values (compute_projection.rs:82). */
+int artefact_hip_plan_download_cos(artefact_hip_plan *plan, float *const *out_cos);
+/* Per-channel gradient norms of the last iteration run (compute_step.rs:54-58). */
+int artefact_hip_plan_last_norms(artefact_hip_plan *plan, float *out_norms);
+/* Gradient of the first step from the start image, before the norm
+ * (compute_step.rs:21-48): prob + TV + second-order term, nchannel planes of max_w*max_h. */
+int artefact_hip_plan_first_gradient(artefact_hip_plan *plan, float *const *out_grad);
+/* Runs the iteration loop without a graph, bracketing every kernel with HIP events
+ * on the plan's stream: avg_ms[0] = mean k_gradient duration, avg_ms[1] = mean
+ * k_project duration, avg_ms[2] = the whole loop, all in milliseconds.  Like
+ * iterate, it consumes the initialised state.  No reference counterpart: the
+ * reference's only timing hook is its benchmark exit (lib.rs:155-157). */
+int artefact_hip_plan_profile(artefact_hip_plan *plan, float *avg_ms);
+/* The plan's HIP stream (hipStream_t), for callers that record events on it. */
+void *artefact_hip_plan_stream(artefact_hip_plan *plan);
+/* Enables (1, default) or disables (0) capturing the iteration loop in a hipGraph. */
+int artefact_hip_plan_set_graph(artefact_hip_plan *plan, int enable);
+void artefact_hip_plan_destroy(artefact_hip_plan *plan);
+
+const char *artefact_hip_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

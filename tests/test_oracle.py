@@ -1,0 +1,434 @@
+"""Pins of the CPU oracle (oracle/oracle.c) -- CPU only.
+
+The reference has no tests or golden vectors for this path, so every function
+of the oracle is pinned by something independent of its own text: a closed
+form, a finite-difference gradient of the (separately computed) f64 objective,
+an exact rational sum, a brute-force permutation, a pure-Python loop
+restatement written from the reference files, or an invariant.
+"""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle
+from artefact_amd import synth
+from artefact_amd.types import Coefficient
+
+f32 = np.float32
+D = synth.dct_matrix()
+
+
+def rand_blocks(n, seed, scale=50.0):
+    return np.random.default_rng(seed).normal(size=(n, 64)) * scale
+
+
+# ---- D1: 8x8 DCT / IDCT (utils/dct.rs) -------------------------------------
+
+def test_dct_closed_form_f64():
+    X = rand_blocks(20, 1).reshape(20, 8, 8)
+    got = oracle.dct8x8(X.reshape(20, 64), dtype=np.float64).reshape(20, 8, 8)
+    ref = np.einsum("ux,bxy,vy->buv", D, X, D)
+    assert np.abs(got - ref).max() < 1e-11
+
+
+def test_idct_closed_form_f64():
+    X = rand_blocks(20, 2).reshape(20, 8, 8)
+    got = oracle.idct8x8(X.reshape(20, 64), dtype=np.float64).reshape(20, 8, 8)
+    ref = np.einsum("ux,buv,vy->bxy", D, X, D)
+    assert np.abs(got - ref).max() < 1e-11
+
+
+def test_dct_f32_close_to_closed_form_and_round_trip():
+    X = rand_blocks(50, 3).astype(f32)
+    got = oracle.dct8x8(X)
+    ref = np.einsum("ux,bxy,vy->buv", D, X.reshape(-1, 8, 8).astype(np.float64), D).reshape(-1, 64)
+    assert np.abs(got - ref).max() < 1e-3
+    back = oracle.idct8x8(got)
+    assert np.abs(back - X).max() < 1e-3
+
+
+def test_dct_transposed_operand_would_fail():
+    """A transposed operand (rows and columns swapped) is caught: the test above
+    is sensitive to the orientation of the output index (v*8+u)."""
+    X = rand_blocks(5, 4).reshape(5, 8, 8)
+    got = oracle.dct8x8(X.reshape(5, 64), dtype=np.float64).reshape(5, 8, 8)
+    wrong = np.einsum("ux,bxy,vy->bvu", D, X, D)
+    assert np.abs(got - wrong).max() > 1.0
+
+
+# ---- D2: boxing / unboxing (utils/boxing.rs) -------------------------------
+
+@pytest.mark.parametrize("bw,bh", [(1, 1), (3, 2), (4, 5)])
+def test_boxing_permutation(bw, bh):
+    rw = bw * 8
+    plane = np.arange(rw * bh * 8, dtype=f32)
+    boxed = oracle.boxing(plane, rw, bw, bh)
+    # brute force: block index by*bw+bx, natural order inside a block
+    expect = np.empty_like(plane)
+    i = 0
+    for by in range(bh):
+        for bx in range(bw):
+            for iy in range(8):
+                for ix in range(8):
+                    expect[i] = plane[(by * 8 + iy) * rw + bx * 8 + ix]
+                    i += 1
+    assert np.array_equal(boxed, expect)
+    assert np.array_equal(oracle.unboxing(boxed, rw, bw, bh), plane)
+
+
+# ---- A3/A4: start image ----------------------------------------------------
+
+def small_coef(bw, bh, hs, vs, seed, q=None):
+    rng = np.random.default_rng(seed)
+    c = rng.integers(-30, 30, size=bw * bh * 64).astype(np.int16)
+    c[::64] = rng.integers(-60, 60, size=bw * bh)
+    qt = synth.quant_table(60) if q is None else q
+    return Coefficient(bw * 8, bh * 8, bw, bh, hs, vs, c, qt)
+
+
+@pytest.mark.parametrize("hs,vs", [(1, 1), (2, 2), (2, 1), (1, 2)])
+def test_init_corrected_closed_form(hs, vs):
+    k = small_coef(3, 2, hs, vs, 5)
+    W, H = k.rounded_px_w * hs, k.rounded_px_h * vs
+    got = oracle.init_plane(k, W, H).reshape(H, W)
+    blocks = (k.dct_coefs.reshape(-1, 8, 8).astype(np.float64) * k.quant_table.reshape(8, 8))
+    pix = np.einsum("ux,buv,vy->bxy", D, blocks, D)
+    img = pix.reshape(k.block_h, k.block_w, 8, 8).transpose(0, 2, 1, 3).reshape(k.rounded_px_h, k.rounded_px_w)
+    up = np.repeat(np.repeat(img, vs, axis=0), hs, axis=1)
+    assert np.abs(got - up).max() < 1e-3
+
+
+def test_init_clamps_to_last_row_and_column():
+    """get_fdata clamps the source index at rounded_px_h/w - 1 (coef.rs:157-160)."""
+    k = small_coef(2, 2, 1, 1, 6)
+    W, H = 24, 32
+    got = oracle.init_plane(k, W, H).reshape(H, W)
+    base = oracle.init_plane(k, 16, 16).reshape(16, 16)
+    assert np.array_equal(got[:16, :16], base)
+    assert np.array_equal(got[20, :16], base[15])
+    assert np.array_equal(got[:16, 23], base[:, 15])
+
+
+def test_init_scalar_quirk_brute_force():
+    """The default scalar pipeline unboxes the whole buffer after every block
+    (pipeline_scalar/coef.rs:29-50); restated here as a Python loop."""
+    k = small_coef(3, 2, 1, 1, 7)
+    W, H, M = k.rounded_px_w, k.rounded_px_h, k.rounded_px_count
+    img = np.zeros(M, dtype=f32)
+    q = k.quant_table.astype(f32)
+    for i in range(k.block_count):
+        img[i * 64:(i + 1) * 64] = k.dct_coefs[i * 64:(i + 1) * 64].astype(f32) * q
+        img[i * 64:(i + 1) * 64] = oracle.idct8x8(img[i * 64:(i + 1) * 64])[0]
+        clone = img.copy()
+        idx = 0
+        for by in range(k.block_h):
+            for bx in range(k.block_w):
+                for iy in range(8):
+                    for ix in range(8):
+                        img[(by * 8 + iy) * W + bx * 8 + ix] = clone[idx]
+                        idx += 1
+    got = oracle.init_plane(k, W, H, init_mode="scalar_quirk")
+    assert np.array_equal(got, img)
+    assert not np.array_equal(got, oracle.init_plane(k, W, H))
+
+
+# ---- A2/A5: FISTA scalars --------------------------------------------------
+
+def test_fista_scalars_closed_form():
+    nt, fa, step = oracle.fista_scalars(100, 2048 * 1536)
+    t = 1.0
+    for k in range(100):
+        t_next = (1.0 + np.sqrt(1.0 + 4.0 * t * t)) / 2.0
+        assert abs(nt[k] - t_next) <= 4e-7 * t_next
+        assert abs(fa[k] - (t - 1.0) / t_next) <= 1e-6
+        t = float(nt[k])
+    assert fa[0] == 0.0
+    assert abs(step - np.sqrt(2048 * 1536) / 2.0 / np.sqrt(101.0)) < 1e-3
+
+
+def test_midpoint_widened():
+    assert oracle.midpoint(1.0, 2.0) == 1.5
+    a, b = f32(3.4028235e38), f32(3.4028235e38)
+    assert oracle.midpoint(a, b) == a  # no overflow: computed in f64
+    assert oracle.midpoint(f32(1e-45), f32(0)) in (0.0, f32(1e-45))
+
+
+# ---- T1/T2/A7: gradients -----------------------------------------------------
+
+def objective_parts(k_list, weight, pweight, W, H, planes):
+    return oracle.objective(k_list, weight, pweight, W, H, planes)
+
+
+def fd_gradient(k_list, weight, pweight, W, H, planes, pts, h=1e-4, part=None):
+    out = []
+    for c, idx in pts:
+        xp = [p.astype(np.float64).copy() for p in planes]
+        xm = [p.astype(np.float64).copy() for p in planes]
+        xp[c][idx] += h
+        xm[c][idx] -= h
+        fp, tp = oracle.objective(k_list, weight, pweight, W, H, xp)
+        fm, tm = oracle.objective(k_list, weight, pweight, W, H, xm)
+        out.append(((tp[part] - tm[part]) if part is not None else (fp - fm)) / (2 * h))
+    return np.array(out)
+
+
+def smooth_planes(C, H, W, seed):
+    rng = np.random.default_rng(seed)
+    return [(rng.normal(size=(H, W)) * 20).astype(f32).reshape(-1) for _ in range(C)]
+
+
+def test_tv_gradient_matches_finite_differences():
+    C, W, H = 3, 16, 16
+    ks = [small_coef(2, 2, 1, 1, s) for s in range(C)]
+    planes = smooth_planes(C, H, W, 1)
+    cos = [np.zeros(W * H, f32)] * C
+    g, _, _ = oracle.gradient(ks, 0.0, [0, 0, 0], W, H, planes, cos)
+    pts = [(c, i) for c in range(C) for i in (0, 5, 17, 100, 255, 240, 15)]
+    fd = fd_gradient(ks, 0.0, [0, 0, 0], W, H, planes, pts, part=0)
+    got = np.array([g[c][i] for c, i in pts])
+    assert np.allclose(got, fd, rtol=2e-3, atol=2e-4)
+
+
+def test_tgv_gradient_interior_matches_finite_differences():
+    C, W, H = 3, 16, 16
+    ks = [small_coef(2, 2, 1, 1, s) for s in range(C)]
+    planes = smooth_planes(C, H, W, 2)
+    cos = [np.zeros(W * H, f32)] * C
+    g0, _, _ = oracle.gradient(ks, 0.0, [0, 0, 0], W, H, planes, cos)
+    g1, _, _ = oracle.gradient(ks, 0.9, [0, 0, 0], W, H, planes, cos)
+    # interior pixels only: the reference's border updates are code-defined, not the adjoint
+    pts = [(c, y * W + x) for c in range(C) for (y, x) in ((3, 3), (5, 9), (8, 8), (12, 4), (10, 13))]
+    fd = fd_gradient(ks, 0.9, [0, 0, 0], W, H, planes, pts, part=1)
+    got = np.array([g1[c][i] - g0[c][i] for c, i in pts])
+    assert np.allclose(got, fd, rtol=5e-3, atol=5e-4)
+
+
+def test_prob_gradient_matches_finite_differences_luma():
+    W = H = 16
+    k = small_coef(2, 2, 1, 1, 9)
+    x = smooth_planes(1, H, W, 3)[0]
+    cos = oracle.dct8x8(oracle.boxing(x, W, 2, 2)).reshape(-1)
+    pw = 0.01
+    g1, _, _ = oracle.gradient([k], 0.0, [pw, 0, 0], W, H, [x], [cos])
+    g0, _, _ = oracle.gradient([k], 0.0, [0, 0, 0], W, H, [x], [cos])
+    pts = [(0, i) for i in (0, 9, 77, 130, 255)]
+    fd = fd_gradient([k], 0.0, [pw, 0, 0], W, H, [x], pts, part=2)
+    got = np.array([g1[0][i] - g0[0][i] for _, i in pts])
+    assert np.allclose(got, fd, rtol=2e-3, atol=2e-4)
+
+
+def scatter_gradient_python(planes, W, H, weight, C):
+    """Pure-Python loop restatement of compute_step_tv.rs:89-153 and
+    compute_step_tv2.rs:184-289, in np.float32 scalars (tiny planes only)."""
+    g = [np.zeros(W * H, f32) for _ in range(C)]
+    fd = planes
+    a = f32(1) / np.sqrt(f32(C))
+    dxs = [np.zeros(W * H, f32) for _ in range(C)]
+    dys = [np.zeros(W * H, f32) for _ in range(C)]
+    for y in range(H):
+        for x in range(W):
+            p = y * W + x
+            gx = [f32(0)] * C
+            gy = [f32(0)] * C
+            for c in range(C):
+                if x < W - 1:
+                    gx[c] = fd[c][p + 1] - fd[c][p]
+                if y < H - 1:
+                    gy[c] = fd[c][p + W] - fd[c][p]
+            n = f32(0)
+            for c in range(C):
+                n = f32(n + gx[c] * gx[c])
+                n = f32(n + gy[c] * gy[c])
+            n = np.sqrt(n)
+            if n != 0:
+                for c in range(C):
+                    g[c][p] += (a * -(gx[c] + gy[c])) / n
+                    if x < W - 1:
+                        g[c][p + 1] += (a * gx[c]) / n
+                    if y < H - 1:
+                        g[c][p + W] += (a * gy[c]) / n
+            for c in range(C):
+                dxs[c][p] = gx[c]
+                dys[c][p] = gy[c]
+    a2 = (f32(weight) / np.sqrt(f32(2))) / np.sqrt(f32(C))
+    for y in range(H):
+        for x in range(W):
+            p = y * W + x
+            gxx = [f32(0)] * C
+            gyy = [f32(0)] * C
+            sym = [f32(0)] * C
+            for c in range(C):
+                gxy = f32(0)
+                gyx = f32(0)
+                if x != 0:
+                    gxx[c] = dxs[c][p] - dxs[c][p - 1]
+                    gyx = dys[c][p] - dys[c][p - 1]
+                if y != 0:
+                    gyy[c] = dys[c][p] - dys[c][p - W]
+                    gxy = dxs[c][p] - dxs[c][p - W]
+                sym[c] = f32((float(gxy) + float(gyx)) / 2.0)
+            n2 = f32(0)
+            for c in range(C):
+                n2 = f32(n2 + (gyy[c] * gyy[c] + (f32(2) * (sym[c] * sym[c]) + gxx[c] * gxx[c])))
+            n2 = np.sqrt(n2)
+            if n2 == 0:
+                continue
+            for c in range(C):
+                xx, yy, s = gxx[c], gyy[c], sym[c]
+                gc = g[c]
+                gc[p] += a2 * (-(f32(2) * yy + (f32(2) * xx + f32(2) * s)) / n2)
+                if x > 0:
+                    gc[p - 1] += a2 * ((s + xx) / n2)
+                if x < W - 1:
+                    gc[p + 1] += a2 * ((s + xx) / n2)
+                if y > 0:
+                    gc[p - W] += a2 * ((yy + s) / n2)
+                if y < H - 1:
+                    gc[p + W] += a2 * ((yy + s) / n2)
+                if x < W - 1 and y > 0:
+                    gc[p - W + 1] += a2 * ((-s) / n2)
+                if x > 0 and y < H - 1:
+                    gc[p + W - 1] += a2 * ((-s) / n2)
+    return g
+
+
+@pytest.mark.parametrize("W,H,C", [(8, 8, 3), (16, 8, 1), (16, 16, 3)])
+def test_tv_tgv_scatter_brute_force(W, H, C):
+    ks = [small_coef(W // 8, H // 8, 1, 1, s) for s in range(C)]
+    planes = smooth_planes(C, H, W, 4)
+    cos = [np.zeros(W * H, f32)] * C
+    g, _, _ = oracle.gradient(ks, 0.3, [0, 0, 0], W, H, planes, cos)
+    ref = scatter_gradient_python(planes, W, H, 0.3, C)
+    for a, b in zip(g, ref):
+        assert np.array_equal(a, b)
+
+
+def test_prob_zero_on_first_iteration():
+    wl = synth.workload("tgv_420_32x32", seed=1)
+    W, H = wl["max_w"], wl["max_h"]
+    init = [oracle.init_plane(k, W, H) for k in wl["coefs"]]
+    cq = [k.dct_coefs.astype(f32) * np.tile(k.quant_table.astype(f32), k.block_count) for k in wl["coefs"]]
+    g1, _, _ = oracle.gradient(wl["coefs"], 0.3, [0.5, 0.5, 0.5], W, H, init, cq)
+    g0, _, _ = oracle.gradient(wl["coefs"], 0.3, [0, 0, 0], W, H, init, cq)
+    for a, b in zip(g1, g0):
+        assert np.array_equal(a, b)
+
+
+# ---- P1: projection ----------------------------------------------------------
+
+@pytest.mark.parametrize("hs,vs", [(1, 1), (2, 2), (2, 1)])
+def test_projection_invariants(hs, vs):
+    k = small_coef(3, 2, hs, vs, 11)
+    W, H = k.rounded_px_w * hs, k.rounded_px_h * vs
+    rng = np.random.default_rng(12)
+    x = (oracle.init_plane(k, W, H) + rng.normal(size=W * H) * 15).astype(f32)
+    p1, cos1 = oracle.projection(k, W, H, x)
+    p2, cos2 = oracle.projection(k, W, H, p1)
+    assert np.abs(p2 - p1).max() < 1e-3  # idempotent up to f32 round-off
+    # re-quantisation of the projected image
+    x64 = p1.reshape(H, W).astype(np.float64)
+    if hs * vs > 1:
+        x64 = x64.reshape(H // vs, vs, W // hs, hs).mean(axis=(1, 3))
+    blocks = x64.reshape(k.block_h, 8, k.block_w, 8).transpose(0, 2, 1, 3)
+    coef = np.einsum("ux,abxy,vy->abuv", D, blocks, D).reshape(-1, 64)
+    dev = np.abs(coef / k.quant_table.astype(np.float64) - k.dct_coefs.reshape(-1, 64))
+    assert dev.max() <= 0.5 + 1e-4
+    # cos holds exactly the clamped values, inside the box
+    q = np.tile(k.quant_table.astype(f32), k.block_count)
+    c = k.dct_coefs.astype(f32)
+    assert np.all(cos1 >= (c - f32(0.5)) * q) and np.all(cos1 <= (c + f32(0.5)) * q)
+    if hs * vs > 1:
+        # the part orthogonal to the subsampling vector is untouched
+        def detail(v):
+            a = v.reshape(H, W).astype(np.float64)
+            m = a.reshape(H // vs, vs, W // hs, hs).mean(axis=(1, 3))
+            return a - np.repeat(np.repeat(m, vs, axis=0), hs, axis=1)
+        assert np.abs(detail(p1) - detail(x)).max() < 1e-3
+
+
+# ---- norm (compute_step.rs:54-58) ------------------------------------------
+
+def test_norm_f64_mode_equals_exact_sum():
+    wl = synth.workload("tgv_420_32x32", seed=3)
+    W, H = wl["max_w"], wl["max_h"]
+    _, norms = oracle.compute(wl["coefs"], 0.3, wl["pweight"], 1, W, H, norm_mode="f64", return_norms=True)
+    init = [oracle.init_plane(k, W, H) for k in wl["coefs"]]
+    cq = [k.dct_coefs.astype(f32) * np.tile(k.quant_table.astype(f32), k.block_count) for k in wl["coefs"]]
+    g, _, _ = oracle.gradient(wl["coefs"], 0.3, wl["pweight"], W, H, init, cq)
+    for c in range(3):
+        exact = sum(Fraction(float(v)) ** 2 for v in g[c])
+        assert norms[0, c] == f32(np.sqrt(float(exact)))
+
+
+def test_norm_reference_mode_within_f32_fold_bound():
+    wl = synth.workload("tgv_420_32x32", seed=3)
+    W, H = wl["max_w"], wl["max_h"]
+    _, n_ref = oracle.compute(wl["coefs"], 0.3, wl["pweight"], 1, W, H, return_norms=True)
+    _, n_f64 = oracle.compute(wl["coefs"], 0.3, wl["pweight"], 1, W, H, norm_mode="f64", return_norms=True)
+    N = W * H
+    bound = N * 2.0 ** -24  # relative error bound of a sequential f32 fold (sqrt halves it)
+    assert np.all(np.abs(n_ref[0] - n_f64[0]) <= bound * n_f64[0] + 1e-6)
+
+
+# ---- A2: the loop's special cases --------------------------------------------
+
+def test_iterations_zero_returns_start_image():
+    wl = synth.workload("tgv_420_32x32", seed=5)
+    W, H = wl["max_w"], wl["max_h"]
+    out = oracle.compute(wl["coefs"], 0.3, wl["pweight"], 0, W, H)
+    for o, k in zip(out, wl["coefs"]):
+        assert np.array_equal(o, oracle.init_plane(k, W, H))
+
+
+def test_all_zero_coefficients_stay_zero():
+    k = Coefficient(16, 16, 2, 2, 1, 1, np.zeros(256, np.int16), synth.quant_table(50))
+    out = oracle.compute([k], 0.3, [0.001], 10, 16, 16)
+    assert np.all(out[0] == 0)
+
+
+def test_weight_zero_is_tv_loop():
+    """TV mode is the TGV loop with weight 0: every second-order term is +-0."""
+    wl = synth.workload("tgv_420_32x32", seed=6)
+    W, H = wl["max_w"], wl["max_h"]
+    a = oracle.compute(wl["coefs"], 0.0, wl["pweight"], 5, W, H)
+    g0, _, _ = oracle.gradient(wl["coefs"], 0.0, [0, 0, 0], W, H, a, [np.zeros(k.rounded_px_count, f32) for k in wl["coefs"]])
+    # the second-order part contributes nothing at weight 0
+    gtv, _, _ = oracle.gradient(wl["coefs"], -0.0, [0, 0, 0], W, H, a, [np.zeros(k.rounded_px_count, f32) for k in wl["coefs"]])
+    for x, y in zip(g0, gtv):
+        assert np.array_equal(x, y)
+
+
+def test_f32_loop_tracks_f64_loop():
+    wl = synth.workload("tgv_420_32x32", seed=7)
+    W, H = wl["max_w"], wl["max_h"]
+    a = oracle.compute(wl["coefs"], 0.3, wl["pweight"], 20, W, H)
+    b = oracle.compute_f64(wl["coefs"], 0.3, wl["pweight"], 20, W, H)
+    fa, _ = oracle.objective(wl["coefs"], 0.3, wl["pweight"], W, H, a)
+    fb, _ = oracle.objective(wl["coefs"], 0.3, wl["pweight"], W, H, b)
+    assert abs(fa - fb) / fb < 1e-3
+
+
+# ---- O: objective ------------------------------------------------------------
+
+def test_objective_closed_forms():
+    W = H = 16
+    k = Coefficient(16, 16, 2, 2, 1, 1, np.zeros(256, np.int16), synth.quant_table(50))
+    const = [np.full(W * H, 3.0, f32)]
+    tot, terms = oracle.objective([k], 0.3, [0.0], W, H, const)
+    assert terms[0] == 0 and terms[1] == 0
+    # ramp along x with slope s: dx = s except on the last column, dy = 0
+    s = 0.5
+    ramp = [np.tile(np.arange(W, dtype=f32) * f32(s), H)]
+    tot, terms = oracle.objective([k], 0.3, [0.0], W, H, ramp)
+    assert abs(terms[0] - s * (W - 1) * H) < 1e-9
+    # second differences of a ramp vanish except where the last column resets dx
+    # to 0: gxx = -s on column W-1, so n2 = s there
+    a2 = float((f32(0.3) / np.sqrt(f32(2))) / np.sqrt(f32(1)))
+    assert abs(terms[1] - a2 * s * H) < 1e-6
+    # prob term: constant plane v, DC coefficient 0 -> only DC deviates: (8 v / q0)^2 per block
+    pw = f32(0.01)
+    tot, terms = oracle.objective([k], 0.3, [pw], W, H, const)
+    ap = float(pw * f32(2) * f32(255) * np.sqrt(f32(2)))
+    q0 = float(k.quant_table[0])
+    assert abs(terms[2] - 0.5 * ap * 4 * (8 * 3.0 / q0) ** 2) < 1e-6
