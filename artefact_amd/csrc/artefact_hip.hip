@@ -1,23 +1,29 @@
 // artefact_hip: the reconstruction loop of artefact-lib as hand-written gfx950 kernels.
 //
-// One iteration of backend/artefact-lib/pipeline_scalar/mod.rs:49-75 is two kernels:
+// One iteration of backend/artefact-lib/pipeline_scalar/mod.rs:49-75 is three kernels:
 //
-//   k_gradient  FISTA extrapolation (mod.rs:54-59) + prob gradient (compute_step_prob.rs)
-//               + TV gradient (compute_step_tv.rs) + second-order gradient
-//               (compute_step_tv2.rs), in gather form over LDS-staged 16x64 tiles with a
-//               2-pixel halo; per-tile partial sums of g^2 in double-double, reduced to
-//               the per-channel gradient norm by the last tile to finish.
+//   k_gradient  FISTA extrapolation (mod.rs:54-59) + TV gradient (compute_step_tv.rs) +
+//               second-order gradient (compute_step_tv2.rs) + the prob gradient
+//               (compute_step_prob.rs, read from the plane k_project left behind).
+//               A register-streaming stencil: each lane owns one image column and walks
+//               down a strip of rows; horizontal neighbours come from DPP lane shifts,
+//               so there is no LDS staging and no barrier in the stencil.  Per-workgroup
+//               sums of g^2 in double-double go to a partials array.
+//   k_finalize  One workgroup reduces the partials, in a fixed order, to the per-channel
+//               gradient norm (compute_step.rs:54-58).
 //   k_project   FISTA extrapolation again (recomputed, never stored), the normalised
 //               descent step (compute_step.rs:61-65) and the box projection
 //               (compute_projection.rs): chroma box mean, 8x8 DCT, clamp to the
-//               quantisation interval, coefficient write-back, 8x8 IDCT, mean re-add.
+//               quantisation interval, 8x8 IDCT, mean re-add.  It also evaluates the next
+//               iteration's prob term, alpha_p * IDCT((cos - c q) / q^2), while the
+//               clamped coefficients are still in LDS, so the coefficient estimate itself
+//               only goes to HBM on the last iteration.
 //
-// Both kernels reproduce the reference's f32 operation order (no contraction: built
-// with -ffp-contract=off; IEEE division and square root), so their output equals the
-// CPU restatement bit for bit once the gradient norm is computed the same way.  The
-// gradient norm is the one place where the order differs: the reference folds
-// N squares sequentially in f32 (compute_step.rs:54-58); here the squares are summed
-// exactly enough in double-double and rounded once, an order-independent value.
+// Both kernels reproduce the reference's f32 operation order (no contraction: built with
+// -ffp-contract=off), so their output equals the CPU restatement bit for bit once the
+// gradient norm is computed the same way.  The gradient norm is the one place where the
+// order differs: the reference folds N squares sequentially in f32 (compute_step.rs:54-58);
+// here the squares are summed in double-double and rounded once.
 //
 // Gather form of the scatter loops.  The reference scatters from every source pixel p
 // into up to 3 (TV) and 7 (second-order) neighbours, in raster order of p.  Every
@@ -27,6 +33,17 @@
 // operands -0.0), so adding a zero term is a no-op: sources outside the image and
 // sources whose norm is 0 contribute an explicit 0 instead of a branch, and every
 // border rule of compute_step_tv2.rs:96-135 reduces to "the source pixel exists".
+//
+// Correctly rounded f32 division by a shared denominator.  The reference divides each
+// numerator by the joint norm (compute_step_tv.rs:136-143, compute_step_tv2.rs:256-286),
+// by q^2 and by the gradient norm.  Here a/b is evaluated as (float)((double)a * r) with
+// r a reciprocal of b accurate to ~2^-52.  That equals the IEEE f32 quotient: for f32
+// a, b the exact quotient is never an f32 rounding midpoint and lies at least 2^-49
+// (relative) away from every midpoint (a/b - m = (a - m b)/b with m = M 2^mu, M an odd
+// 25-bit integer, and |a - m b| >= 2^e gives a relative gap >= 1/(M B) > 2^-49 for the
+// 24-bit significand B of b), while the f64 product is within ~2^-52 of a/b, so rounding
+// it to f32 picks the same neighbour.  All quotients here are finite: the iterates stay
+// far below 2^60 in magnitude (int16 coefficients, u16 tables, bounded steps).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -40,15 +57,13 @@
 
 namespace {
 
-constexpr int TW = 64;          // tile width (full-resolution pixels)
-constexpr int TH = 16;          // tile height
 constexpr int NT = 256;         // threads per workgroup
-constexpr int HX = TW + 4;      // extrapolated-image tile with a 2-pixel halo
-constexpr int HY = TH + 4;
-constexpr int RX = TW + 2;      // gradient source positions: tile + 1-pixel ring
-constexpr int RY = TH + 2;
+constexpr int WAVES = NT / 64;  // waves per workgroup
+constexpr int KG_COLS = 60;     // k_gradient: output columns per wave (64 lanes, 2+2 halo)
+constexpr int TW = 64;          // k_project tile width (full-resolution pixels)
+constexpr int TH = 16;          // k_project tile height
 constexpr int BSTR = 72;        // LDS floats per 8x8 block: 8 rows of 9 (bank spread)
-constexpr int MAXBLK = 16;      // 8x8 blocks per tile at most (luma: 2 x 8)
+constexpr int MAXBLK = 16;      // 8x8 blocks per k_project tile at most (2 x 8)
 
 // Ooura shrtdct 8-point constants, backend/artefact-lib/utils/dct.rs:20-27.
 constexpr float C8_1R = 0.49039264020161522456f;
@@ -61,7 +76,7 @@ constexpr float C8_4R = 0.35355339059327376220f;
 constexpr float W8_4R = 0.70710678118654752440f;
 
 // One 8-point pass of idct8x8s (utils/dct.rs:30-98); v[k] is element k of the column
-// (first pass) or row (second pass).
+// (first pass) or row (second pass).  a + -(b) of the reference is a - b in IEEE.
 __device__ __forceinline__ void idct8(float v[8])
 {
     float x1r = C8_1R * v[1] + C8_1I * v[7];
@@ -140,6 +155,33 @@ __device__ __forceinline__ float clamp_ref(float v, float lo, float hi)
     return v;
 }
 
+// Reciprocal of a positive finite f32 in f64, accurate to ~2^-52: hardware estimate
+// (v_rcp_f64) refined by two Newton steps.
+__device__ __forceinline__ double recip64(float b)
+{
+    const double d = (double)b;
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    return r;
+}
+
+// a / b for f32 a and b, given r ~ 1/b (see the header comment): bit-identical to IEEE.
+__device__ __forceinline__ float qdiv(float a, double r) { return (float)((double)a * r); }
+
+// Value of the same register in lane - 1 (wave_shr:1) / lane + 1 (wave_shl:1); the edge
+// lane reads 0.  Both are DPP modifiers on a v_mov, not LDS traffic.
+__device__ __forceinline__ float from_left(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float from_right(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+
 // Double-double accumulation (Knuth two-sum), for an order-insensitive sum of squares.
 struct DD {
     double hi, lo;
@@ -171,276 +213,278 @@ __device__ __forceinline__ DD dd_shfl_xor(DD a, int m)
     return r;
 }
 
+enum ProjMode : uint32_t { PM_FULL = 0, PM_SUB11 = 1, PM_SUB22 = 2, PM_SUB21 = 3, PM_SUB12 = 4 };
+
 struct ChanDev {
     const int16_t *coef;  // M_c, block order
-    float *cos;           // M_c, block order
+    float *cos;           // M_c, block order (written on the last iteration)
+    float *prob;          // M_c, component raster: alpha_p * IDCT((cos - c q) / q^2)
     const float *quant;   // 64, natural order, as f32
-    const float *qq;      // 64, quant*quant rounded to f32 (compute_step_prob.rs:29)
-    uint32_t rw, rh, bw, bh, hs, vs, resample;
+    const double *rqq;    // 64, 1 / (quant*quant rounded to f32), correctly rounded f64
+    uint32_t rw, rh, bw, bh, hs, vs, resample, mode;
     float alpha_p;        // pweight*2*255*sqrt(2) (compute_step.rs:30); 0 disables the term
 };
 
 struct IterArgs {
-    uint32_t W, H, C;
+    uint32_t W, H, C, write_cos;
+    uint32_t prob_bytes;     // bytes of the prob arena (all channels)
     float factor, step, alpha_tv, alpha_tgv;
     const float *x[3];
     const float *xp[3];
     float *g[3];
     float *xnew[3];
     ChanDev ch[3];
-    double *partials;        // [workgroups][3][2]
-    unsigned int *counter;   // tiles finished; reset by the last one
+    double *partials;        // [workgroups][3][2]: double-double sums of g^2
     float *norms;            // [3]
+    double *rnorm;           // [3] 1/norm, correctly rounded f64
     double *norm_sq;         // [3]
 };
 
-enum { TVS = 0, TVR, TVD, TGS, TGX, TGY, TGD };
+// Contributions of one source pixel: TV self/right/down, second-order self/x/y/diagonal.
+struct Contrib {
+    float tvs, tvr, tvd, tgs, tgx, tgy, tgd;
+};
 
-// Forward differences at an LDS position of the extrapolated tile, with the
-// reference's edge rule (compute_step_tv.rs:48-60): 0 on the last column / row.
-__device__ __forceinline__ void fwd_diff(const float (*ys)[HX], int sy, int sx, int py, int px, int W, int H,
-                                         float &gx, float &gy)
+// Raw buffer resource over `bytes` bytes: loads outside return 0, stores outside are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, uint32_t bytes)
 {
-    float v = ys[sy][sx];
-    gx = px < W - 1 ? ys[sy][sx + 1] - v : 0.0f;
-    gy = py < H - 1 ? ys[sy + 1][sx] - v : 0.0f;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
-
-// Second-order terms at a source pixel (compute_step_tv2.rs:53-72).
-__device__ __forceinline__ void tgv_terms(const float (*ys)[HX], int sy, int sx, int py, int px, int W, int H,
-                                          float gx, float gy, float &gxx, float &gyy, float &sym)
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {
-    float gyx = 0.0f, gxy = 0.0f;
-    gxx = 0.0f;
-    gyy = 0.0f;
-    if (px != 0) {
-        float lx, ly;
-        fwd_diff(ys, sy, sx - 1, py, px - 1, W, H, lx, ly);
-        gxx = gx - lx;
-        gyx = gy - ly;
-    }
-    if (py != 0) {
-        float ux, uy;
-        fwd_diff(ys, sy - 1, sx, py - 1, px, W, H, ux, uy);
-        gyy = gy - uy;
-        gxy = gx - ux;
-    }
-    sym = midpoint(gxy, gyx);
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
-
-template <bool TGV>
-__global__ __launch_bounds__(NT) void k_gradient(const IterArgs a)
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
 {
-    __shared__ float ys[3][HY][HX];
-    __shared__ float nrm[RY][RX];
-    __shared__ float nrm2[RY][RX];
-    __shared__ float ctb[7][RY][RX];
-    __shared__ float pblk[MAXBLK * BSTR];
-    __shared__ float probv[TH][TW];
-    __shared__ DD red[NT / 64][3];
-    __shared__ int is_last;
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+constexpr uint32_t OOB = 0xFFFFFFF0u;  // a byte offset no plane reaches (planes hold < 2^31 bytes)
 
-    const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    const int W = a.W, H = a.H, C = a.C;
-    const float f = a.factor;
+// k_gradient: C channels, TGV on/off, R output rows per wave.  Every wave walks rows
+// r0-2 .. r0+R+1 of its 64 columns (2 halo columns each side); the row loop is fully
+// unrolled so the three-row window of contributions lives in fixed registers.
+template <int C, bool TGV, int R>
+__global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
+{
+    __shared__ DD red[WAVES][3];
 
-    // 1. Extrapolated image y = FISTA(x, x_prev) on the tile + 2-pixel halo.
-    for (int i = tid; i < HY * HX; i += NT) {
-        const int ry = i / HX, rx = i - ry * HX;
-        const int gy = y0 - 2 + ry, gx = x0 - 2 + rx;
-        const bool in = gy >= 0 && gy < H && gx >= 0 && gx < W;
-        const size_t p = in ? (size_t)gy * W + gx : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int W = a.W, H = a.H;
+    const int x = (int)blockIdx.x * KG_COLS - 2 + lane;
+    const int r0 = ((int)blockIdx.y * WAVES + wave) * R;
+    const bool xin = x >= 0 && x < W;
+    const bool out_lane = lane >= 2 && lane < 62 && xin;
+    const bool xl = x > 0;
+    const bool xr = x < W - 1;
+    const float f = a.factor, atv = a.alpha_tv, atg = a.alpha_tgv;
+    const uint32_t pbytes = (uint32_t)((size_t)W * H * 4);
+
+    // One buffer resource per arena (the C planes of x, x_prev and g are contiguous, the
+    // prob planes too); channel c sits c * pbytes (prob: poff[c]) bytes in.
+    const __amdgpu_buffer_rsrc_t rx = plane_rsrc(a.x[0], pbytes * C);
+    const __amdgpu_buffer_rsrc_t rxp = plane_rsrc(a.xp[0], pbytes * C);
+    const __amdgpu_buffer_rsrc_t rg = plane_rsrc(a.g[0], pbytes * C);
+    const __amdgpu_buffer_rsrc_t rpr = plane_rsrc(a.ch[0].prob, a.prob_bytes);
+    uint32_t poff[C];
+    int hsh[C], vsh[C], crw[C], crh[C];
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-            if (c < C) ys[c][ry][rx] = in ? fista(a.x[c][p], a.xp[c][p], f) : 0.0f;
+    for (int c = 0; c < C; c++) {
+        poff[c] = (uint32_t)((const char *)a.ch[c].prob - (const char *)a.ch[0].prob);
+        hsh[c] = a.ch[c].hs - 1;
+        vsh[c] = a.ch[c].vs - 1;
+        crw[c] = a.ch[c].rw;
+        crh[c] = a.ch[c].rh;
     }
-    __syncthreads();
+    // Byte offset of (row t, this lane's column) in channel 0; OOB outside the image.
+    auto roff = [&](int t) -> uint32_t { return (xin && t >= 0 && t < H) ? (uint32_t)(t * W + x) * 4u : OOB; };
+    auto chan = [&](uint32_t o, int c) -> uint32_t { return o == OOB ? OOB : o + (uint32_t)c * pbytes; };
 
-    // 2. Joint norms n(p) (compute_step_tv.rs:62-68) and n2(p) (compute_step_tv2.rs:76-86)
-    //    at every source position that reaches the tile; 0 outside the image.
-    for (int i = tid; i < RY * RX; i += NT) {
-        const int ry = i / RX, rx = i - ry * RX;
-        const int py = y0 - 1 + ry, px = x0 - 1 + rx;
-        float n = 0.0f, n2 = 0.0f;
-        if (py >= 0 && py < H && px >= 0 && px < W) {
-            const int sy = ry + 1, sx = rx + 1;
-            float nacc = 0.0f, n2acc = 0.0f;
+    float yprev[C], dxprev[C], dxs1[C], dys1[C], nx[C], nxp[C];
+    Contrib slot[3][C];
+    DD acc[C];
 #pragma unroll
-            for (int c = 0; c < 3; c++) {
-                if (c < C) {
-                    float gx, gy;
-                    fwd_diff(ys[c], sy, sx, py, px, W, H, gx, gy);
+    for (int c = 0; c < C; c++) {
+        yprev[c] = dxprev[c] = dxs1[c] = dys1[c] = 0.0f;
+        acc[c] = DD{0.0, 0.0};
+        const uint32_t o = chan(roff(r0 - 2), c);
+        nx[c] = bload(rx, o);
+        nxp[c] = bload(rxp, o);
+    }
+
+#pragma unroll
+    for (int k = 0; k < R + 4; k++) {
+        const int t = r0 - 2 + k;
+        float yt[C], dxt[C];
+#pragma unroll
+        for (int c = 0; c < C; c++) yt[c] = fista(nx[c], nxp[c], f);
+        if (k + 1 < R + 4) {
+            const uint32_t o = roff(t + 1);
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                nx[c] = bload(rx, chan(o, c));
+                nxp[c] = bload(rxp, chan(o, c));
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            const float yr = from_right(yt[c]);
+            dxt[c] = xr ? yr - yt[c] : 0.0f;
+        }
+        if (k >= 1) {
+            const int s = t - 1;  // source row whose dy is now known
+            float dys[C];
+#pragma unroll
+            for (int c = 0; c < C; c++) dys[c] = s < H - 1 ? yt[c] - yprev[c] : 0.0f;
+            if (k >= 2) {
+                const bool valid = xin && s >= 0 && s < H;
+                // Joint norms at source (s, x): compute_step_tv.rs:62-68, compute_step_tv2.rs:53-86.
+                float nacc = 0.0f, n2acc = 0.0f;
+                float gxx[C], gyy[C], sym[C];
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const float gx = dxprev[c], gy = dys[c];
                     nacc += gx * gx;
                     nacc += gy * gy;
+                    gxx[c] = gyy[c] = sym[c] = 0.0f;
                     if (TGV) {
-                        float gxx, gyy, sym;
-                        tgv_terms(ys[c], sy, sx, py, px, W, H, gx, gy, gxx, gyy, sym);
-                        n2acc += gyy * gyy + (2.0f * (sym * sym) + gxx * gxx);
+                        const float lgx = from_left(gx), lgy = from_left(gy);
+                        const float xx = xl ? gx - lgx : 0.0f;
+                        const float yx = xl ? gy - lgy : 0.0f;
+                        const float yy = s > 0 ? gy - dys1[c] : 0.0f;
+                        const float xy = s > 0 ? gx - dxs1[c] : 0.0f;
+                        const float sm = midpoint(xy, yx);
+                        n2acc += yy * yy + (2.0f * (sm * sm) + xx * xx);
+                        gxx[c] = xx;
+                        gyy[c] = yy;
+                        sym[c] = sm;
                     }
                 }
-            }
-            n = sqrtf(nacc);
-            if (TGV) n2 = sqrtf(n2acc);
-        }
-        nrm[ry][rx] = n;
-        nrm2[ry][rx] = n2;
-    }
-    __syncthreads();
-
-    DD acc[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) acc[c] = DD{0.0, 0.0};
-
-    const int tr = tid >> 4, tc = (tid & 15) * 4;
-    const float atv = a.alpha_tv, atg = a.alpha_tgv;
-
-    for (int c = 0; c < C; c++) {
-        const ChanDev &ch = a.ch[c];
-        // 3a. Per-source contributions of channel c (compute_step_tv.rs:74-86,
-        //     compute_step_tv2.rs:96-135), zero where the norm is 0.
-        for (int i = tid; i < RY * RX; i += NT) {
-            const int ry = i / RX, rx = i - ry * RX;
-            const int py = y0 - 1 + ry, px = x0 - 1 + rx;
-            float v[7] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-            if (py >= 0 && py < H && px >= 0 && px < W) {
-                const int sy = ry + 1, sx = rx + 1;
-                float gx, gy;
-                fwd_diff(ys[c], sy, sx, py, px, W, H, gx, gy);
-                const float n = nrm[ry][rx];
-                if (n != 0.0f) {
-                    v[TVS] = (atv * -(gx + gy)) / n;
-                    v[TVR] = (atv * gx) / n;
-                    v[TVD] = (atv * gy) / n;
-                }
+                // A zero norm or a source outside the image contributes nothing: its
+                // reciprocal is set to 0, so every contribution becomes +-0, which adds
+                // nothing to an accumulator that is never -0.
+                const float n = sqrtf(nacc);
+                const double rn = (valid && n != 0.0f) ? recip64(n) : 0.0;
+                double rn2 = 0.0;
                 if (TGV) {
-                    const float n2 = nrm2[ry][rx];
-                    if (n2 != 0.0f) {
-                        float gxx, gyy, s;
-                        tgv_terms(ys[c], sy, sx, py, px, W, H, gx, gy, gxx, gyy, s);
-                        v[TGS] = atg * (-(2.0f * gyy + (2.0f * gxx + 2.0f * s)) / n2);
-                        v[TGX] = atg * ((s + gxx) / n2);
-                        v[TGY] = atg * ((gyy + s) / n2);
-                        v[TGD] = atg * ((-s) / n2);
+                    const float n2 = sqrtf(n2acc);
+                    rn2 = (valid && n2 != 0.0f) ? recip64(n2) : 0.0;
+                }
+                Contrib *cur = slot[k % 3];
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    const float gx = dxprev[c], gy = dys[c];
+                    Contrib kc{0, 0, 0, 0, 0, 0, 0};
+                    kc.tvs = qdiv(atv * -(gx + gy), rn);
+                    kc.tvr = qdiv(atv * gx, rn);
+                    kc.tvd = qdiv(atv * gy, rn);
+                    if (TGV) {
+                        const float xx = gxx[c], yy = gyy[c], sm = sym[c];
+                        kc.tgs = atg * qdiv(-(2.0f * yy + (2.0f * xx + 2.0f * sm)), rn2);
+                        kc.tgx = atg * qdiv(sm + xx, rn2);
+                        kc.tgy = atg * qdiv(yy + sm, rn2);
+                        kc.tgd = atg * qdiv(-sm, rn2);
+                    }
+                    cur[c] = kc;
+                }
+                if (k >= 4) {
+                    // Gather into q = s - 1 in the reference's accumulation order.
+                    const int q = s - 1;
+                    const Contrib *m1 = slot[(k + 2) % 3];  // row q = s - 1
+                    const Contrib *m2 = slot[(k + 1) % 3];  // row q - 1 = s - 2
+                    const bool qin = q < H;
+                    const uint32_t qo = out_lane ? roff(q) : OOB;
+                    const bool acc_on = out_lane && qin;
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        const int cy = q >> vsh[c], cx = x >> hsh[c];
+                        const bool pin = out_lane && cx < crw[c] && cy < crh[c];
+                        const float pv = bload(rpr, pin ? poff[c] + (uint32_t)(cy * crw[c] + cx) * 4u : OOB);
+                        const float left_tvr = from_left(m1[c].tvr);
+                        float gv = 0.0f;
+                        gv += pv;
+                        gv += m2[c].tvd;
+                        gv += left_tvr;
+                        gv += m1[c].tvs;
+                        if (TGV) {
+                            const float right_tgd = from_right(m2[c].tgd);
+                            const float left_tgx = from_left(m1[c].tgx);
+                            const float right_tgx = from_right(m1[c].tgx);
+                            const float left_tgd = from_left(cur[c].tgd);
+                            gv += m2[c].tgy;
+                            gv += right_tgd;
+                            gv += left_tgx;
+                            gv += m1[c].tgs;
+                            gv += right_tgx;
+                            gv += left_tgd;
+                            gv += cur[c].tgy;
+                        }
+                        bstore(rg, chan(qo, c), gv);
+                        const float ga = acc_on ? gv : 0.0f;
+                        dd_add(acc[c], (double)ga * (double)ga);
                     }
                 }
             }
 #pragma unroll
-            for (int k = 0; k < 7; k++) ctb[k][ry][rx] = v[k];
-        }
-        // 3b. Prob term (compute_step_prob.rs:14-58): IDCT of (cos - c*q)/q^2 for the
-        //     component blocks under this tile, replicated onto the full grid.
-        for (int i = tid; i < TH * TW; i += NT) probv[i / TW][i % TW] = 0.0f;
-        const int hs = ch.hs, vs = ch.vs;
-        const int nbx = TW / hs / 8, nby = TH / vs / 8;
-        const int b = tid >> 3, j = tid & 7;
-        const int bx = x0 / hs / 8 + b % nbx, by = y0 / vs / 8 + b / nbx;
-        const bool pvalid = ch.alpha_p != 0.0f && b < nbx * nby && bx < (int)ch.bw && by < (int)ch.bh;
-        float pv[8];
-        if (pvalid) {
-            const size_t base = ((size_t)by * ch.bw + bx) * 64;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                float d = ch.cos[base + k * 8 + j];
-                d -= (float)ch.coef[base + k * 8 + j] * ch.quant[k * 8 + j];
-                d /= ch.qq[k * 8 + j];
-                pv[k] = d;
-            }
-            idct8(pv);
-#pragma unroll
-            for (int k = 0; k < 8; k++) pblk[b * BSTR + k * 9 + j] = pv[k];
-        }
-        __syncthreads();
-        if (pvalid) {
-#pragma unroll
-            for (int k = 0; k < 8; k++) pv[k] = pblk[b * BSTR + j * 9 + k];
-            idct8(pv);
-            const int ty = (by * 8 + j) * vs - y0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const int tx = (bx * 8 + k) * hs - x0;
-                const float val = ch.alpha_p * pv[k];
-                for (int sy = 0; sy < vs; sy++)
-                    for (int sx = 0; sx < hs; sx++) probv[ty + sy][tx + sx] = val;
+            for (int c = 0; c < C; c++) {
+                dxs1[c] = dxprev[c];
+                dys1[c] = dys[c];
             }
         }
-        __syncthreads();
-
-        // 3c. Gather in the reference's accumulation order.
-        float out[4];
-        const int qy = y0 + tr;
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int ry = tr + 1, rx = tc + k + 1;
-            float gv = 0.0f;
-            gv += probv[tr][tc + k];
-            gv += ctb[TVD][ry - 1][rx];
-            gv += ctb[TVR][ry][rx - 1];
-            gv += ctb[TVS][ry][rx];
-            if (TGV) {
-                gv += ctb[TGY][ry - 1][rx];
-                gv += ctb[TGD][ry - 1][rx + 1];
-                gv += ctb[TGX][ry][rx - 1];
-                gv += ctb[TGS][ry][rx];
-                gv += ctb[TGX][ry][rx + 1];
-                gv += ctb[TGD][ry + 1][rx - 1];
-                gv += ctb[TGY][ry + 1][rx];
-            }
-            out[k] = gv;
+        for (int c = 0; c < C; c++) {
+            dxprev[c] = dxt[c];
+            yprev[c] = yt[c];
         }
-        if (qy < H && x0 + tc < W) {
-            *reinterpret_cast<float4 *>(a.g[c] + (size_t)qy * W + x0 + tc) =
-                make_float4(out[0], out[1], out[2], out[3]);
-#pragma unroll
-            for (int k = 0; k < 4; k++) dd_add(acc[c], (double)out[k] * (double)out[k]);
-        }
-        __syncthreads();
     }
 
-    // 4. Per-tile partial sums, then the last tile reduces all partials in a fixed order.
-    const int lane = tid & 63, wave = tid >> 6;
+    // Per-workgroup partial sums; k_finalize reduces them in a fixed order.
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < C; c++) {
         DD v = acc[c];
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) v = dd_merge(v, dd_shfl_xor(v, m));
         if (lane == 0) red[wave][c] = v;
     }
     __syncthreads();
-    const unsigned int nwg = gridDim.x * gridDim.y;
-    const unsigned int wg = blockIdx.y * gridDim.x + blockIdx.x;
-    if (tid == 0) {
-        for (int c = 0; c < 3; c++) {
+    if (threadIdx.x == 0) {
+        const unsigned int wg = blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
             DD v = red[0][c];
-            for (int w = 1; w < NT / 64; w++) v = dd_merge(v, red[w][c]);
-            __hip_atomic_store(&a.partials[(size_t)wg * 6 + c * 2 + 0], v.hi, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&a.partials[(size_t)wg * 6 + c * 2 + 1], v.lo, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __threadfence();
-        const unsigned int prev = atomicAdd(a.counter, 1u);
-        is_last = prev == nwg - 1;
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    DD s[3];
-#pragma unroll
-    for (int c = 0; c < 3; c++) s[c] = DD{0.0, 0.0};
-    for (unsigned int w = tid; w < nwg; w += NT) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            DD v;
-            v.hi = __hip_atomic_load(&a.partials[(size_t)w * 6 + c * 2 + 0], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-            v.lo = __hip_atomic_load(&a.partials[(size_t)w * 6 + c * 2 + 1], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-            s[c] = dd_merge(s[c], v);
+            for (int w = 1; w < WAVES; w++) v = dd_merge(v, red[w][c]);
+            a.partials[(size_t)wg * 6 + c * 2 + 0] = v.hi;
+            a.partials[(size_t)wg * 6 + c * 2 + 1] = v.lo;
         }
     }
+}
+
+// Gradient norm per channel from the workgroup partials (compute_step.rs:54-58): one
+// workgroup, fixed reduction order, so the result is the same on every run.
+constexpr int NF = 1024;
+template <int C>
+__global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigned int nwg, float *norms,
+                                                 double *rnorm, double *norm_sq)
+{
+    __shared__ DD red[NF / 64][3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    DD s[C];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < C; c++) s[c] = DD{0.0, 0.0};
+    for (unsigned int w0 = 0; w0 < nwg; w0 += 4 * NF) {
+        double v[4][2 * C];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const unsigned int w = w0 + u * NF + tid;
+#pragma unroll
+            for (int k = 0; k < 2 * C; k++) v[u][k] = w < nwg ? partials[(size_t)w * 6 + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int c = 0; c < C; c++) s[c] = dd_merge(s[c], DD{v[u][2 * c], v[u][2 * c + 1]});
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
         DD v = s[c];
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) v = dd_merge(v, dd_shfl_xor(v, m));
@@ -450,85 +494,66 @@ __global__ __launch_bounds__(NT) void k_gradient(const IterArgs a)
     if (tid == 0) {
         for (int c = 0; c < C; c++) {
             DD v = red[0][c];
-            for (int w = 1; w < NT / 64; w++) v = dd_merge(v, red[w][c]);
+            for (int w = 1; w < NF / 64; w++) v = dd_merge(v, red[w][c]);
             const double sum = v.hi + v.lo;
-            a.norm_sq[c] = sum;
-            a.norms[c] = (float)sqrt(sum);
+            const float norm = (float)sqrt(sum);
+            norm_sq[c] = sum;
+            norms[c] = norm;
+            rnorm[c] = norm != 0.0f ? 1.0 / (double)norm : 0.0;
         }
-        atomicExch(a.counter, 0u);
     }
 }
 
-__global__ __launch_bounds__(NT) void k_project(const IterArgs a)
+struct ProjShared {
+    float yv[TH][TW + 1];        // descended values, full resolution
+    float mv[TH][TW + 1];        // component means (resampled channels)
+    float blk[MAXBLK * BSTR];    // projection blocks
+    float pbk[MAXBLK * BSTR];    // prob-term blocks
+};
+
+// Box projection of one channel over one 16x64 tile (compute_projection.rs:8-122), plus
+// the prob plane of the next iteration (compute_step_prob.rs:14-33).
+template <int HS, int VS, bool RES>
+__device__ __forceinline__ void project_channel(const IterArgs &a, const ChanDev &ch, float *xnew, ProjShared &sh,
+                                                const float yd[4], int tr, int tc, int x0, int y0, bool live)
 {
-    __shared__ float yv[TH][TW + 1];
-    __shared__ float mv[TH][TW + 1];
-    __shared__ float blk[MAXBLK * BSTR];
-
+    constexpr int CW = TW / HS, CT = TH / VS, NBX = CW / 8, NBY = CT / 8, NB = NBX * NBY;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    const int W = a.W, H = a.H, C = a.C;
-    const int tr = tid >> 4, tc = (tid & 15) * 4;
-    const int qy = y0 + tr, qx = x0 + tc;
-    const bool live = qy < H && qx < W;
-    const float f = a.factor, step = a.step;
+    const int cx0 = x0 / HS, cy0 = y0 / VS;
+    const int rw = ch.rw, rh = ch.rh;
 
-    for (int c = 0; c < C; c++) {
-        const ChanDev &ch = a.ch[c];
-        const float norm = a.norms[c];
-        // Descent step on the extrapolated image (compute_step.rs:61-65).
-        float yd[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (live) {
-            const size_t p = (size_t)qy * W + qx;
-            const float4 x4 = *reinterpret_cast<const float4 *>(a.x[c] + p);
-            const float4 p4 = *reinterpret_cast<const float4 *>(a.xp[c] + p);
-            const float4 g4 = *reinterpret_cast<const float4 *>(a.g[c] + p);
-            const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
-            const float ps[4] = {p4.x, p4.y, p4.z, p4.w};
-            const float gs[4] = {g4.x, g4.y, g4.z, g4.w};
+    if (RES) {
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                float y = fista(xs[k], ps[k], f);
-                if (norm != 0.0f) y -= step * (gs[k] / norm);
-                yd[k] = y;
+        for (int k = 0; k < 4; k++) sh.yv[tr][tc + k] = yd[k];
+        __syncthreads();
+        for (int i = tid; i < CW * CT; i += NT) {
+            const int ly = i / CW, lx = i % CW;
+            if (cx0 + lx < rw && cy0 + ly < rh) {
+                float m = 0.0f;
+#pragma unroll
+                for (int sy = 0; sy < VS; sy++)
+#pragma unroll
+                    for (int sx = 0; sx < HS; sx++) m += sh.yv[ly * VS + sy][lx * HS + sx];
+                m /= (float)(HS * VS);
+                sh.mv[ly][lx] = m;
+                sh.blk[((ly >> 3) * NBX + (lx >> 3)) * BSTR + (ly & 7) * 9 + (lx & 7)] = m;
             }
         }
+    } else {
 #pragma unroll
-        for (int k = 0; k < 4; k++) yv[tr][tc + k] = yd[k];
-        __syncthreads();
+        for (int k = 0; k < 4; k++)
+            sh.blk[((tr >> 3) * NBX + ((tc + k) >> 3)) * BSTR + (tr & 7) * 9 + ((tc + k) & 7)] = yd[k];
+    }
+    __syncthreads();
 
-        // Component-resolution view of the tile (compute_projection.rs:18-61).
-        const int hs = ch.hs, vs = ch.vs;
-        const int cw = TW / hs, chh = TH / vs;
-        const int cx0 = x0 / hs, cy0 = y0 / vs;
-        const int nbx = cw / 8, nby = chh / 8;
-        for (int i = tid; i < cw * chh; i += NT) {
-            const int ly = i / cw, lx = i - ly * cw;
-            if (cx0 + lx < (int)ch.rw && cy0 + ly < (int)ch.rh) {
-                float val;
-                if (ch.resample) {
-                    float m = 0.0f;
-                    for (int sy = 0; sy < vs; sy++)
-                        for (int sx = 0; sx < hs; sx++) m += yv[ly * vs + sy][lx * hs + sx];
-                    m /= (float)(hs * vs);
-                    mv[ly][lx] = m;
-                    val = m;
-                } else {
-                    val = yv[ly][lx];
-                }
-                blk[((ly >> 3) * nbx + (lx >> 3)) * BSTR + (ly & 7) * 9 + (lx & 7)] = val;
-            }
-        }
-        __syncthreads();
-
-        // DCT, clamp to [(c - 1/2) q, (c + 1/2) q], keep cos, IDCT
-        // (compute_projection.rs:63-92), 8 threads per block, column pass first.
+    // Forward DCT (column pass, row pass), clamp, prob difference.
+    {
         const int b = tid >> 3, j = tid & 7;
-        const int bx = cx0 / 8 + b % nbx, by = cy0 / 8 + b / nbx;
-        const bool bvalid = b < nbx * nby && bx < (int)ch.bw && by < (int)ch.bh;
+        const int bx = cx0 / 8 + b % NBX, by = cy0 / 8 + b / NBX;
+        const bool bv = b < NB && bx < (int)ch.bw && by < (int)ch.bh;
+        float *bp = sh.blk + b * BSTR;
         float v[8];
-        float *bp = blk + b * BSTR;
-        if (bvalid) {
+        if (bv) {
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k] = bp[k * 9 + j];
             dct8(v);
@@ -536,25 +561,45 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
             for (int k = 0; k < 8; k++) bp[k * 9 + j] = v[k];
         }
         __syncthreads();
-        if (bvalid) {
+        if (bv) {
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k] = bp[j * 9 + k];
             dct8(v);
             const size_t base = ((size_t)by * ch.bw + bx) * 64 + j * 8;
-            float o[8];
+            const short4 c03 = *reinterpret_cast<const short4 *>(ch.coef + base);
+            const short4 c47 = *reinterpret_cast<const short4 *>(ch.coef + base + 4);
+            const float cf[8] = {(float)c03.x, (float)c03.y, (float)c03.z, (float)c03.w,
+                                 (float)c47.x, (float)c47.y, (float)c47.z, (float)c47.w};
+            float o[8], d[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const float cf = (float)ch.coef[base + k];
                 const float q = ch.quant[j * 8 + k];
-                o[k] = clamp_ref(v[k], (cf - 0.5f) * q, (cf + 0.5f) * q);
+                o[k] = clamp_ref(v[k], (cf[k] - 0.5f) * q, (cf[k] + 0.5f) * q);
+                d[k] = qdiv(o[k] - cf[k] * q, ch.rqq[j * 8 + k]);
             }
-            *reinterpret_cast<float4 *>(ch.cos + base) = make_float4(o[0], o[1], o[2], o[3]);
-            *reinterpret_cast<float4 *>(ch.cos + base + 4) = make_float4(o[4], o[5], o[6], o[7]);
+            if (a.write_cos) {
+                *reinterpret_cast<float4 *>(ch.cos + base) = make_float4(o[0], o[1], o[2], o[3]);
+                *reinterpret_cast<float4 *>(ch.cos + base + 4) = make_float4(o[4], o[5], o[6], o[7]);
+            }
+            float *pp = sh.pbk + b * BSTR;
 #pragma unroll
-            for (int k = 0; k < 8; k++) bp[j * 9 + k] = o[k];
+            for (int k = 0; k < 8; k++) {
+                bp[j * 9 + k] = o[k];
+                pp[j * 9 + k] = d[k];
+            }
         }
         __syncthreads();
-        if (bvalid) {
+    }
+
+    // Two IDCTs at once: threads 0..127 the projected block, 128..255 the prob block.
+    {
+        const int half = tid >> 7, t2 = tid & 127;
+        const int b = t2 >> 3, j = t2 & 7;
+        const int bx = cx0 / 8 + b % NBX, by = cy0 / 8 + b / NBX;
+        const bool bv = b < NB && bx < (int)ch.bw && by < (int)ch.bh && (half == 0 || ch.alpha_p != 0.0f);
+        float *bp = (half ? sh.pbk : sh.blk) + b * BSTR;
+        float v[8];
+        if (bv) {
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k] = bp[k * 9 + j];
             idct8(v);
@@ -562,31 +607,85 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
             for (int k = 0; k < 8; k++) bp[k * 9 + j] = v[k];
         }
         __syncthreads();
-        if (bvalid) {
+        if (bv) {
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k] = bp[j * 9 + k];
             idct8(v);
+            if (half == 0) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) bp[j * 9 + k] = v[k];
-        }
-        __syncthreads();
-
-        // Unbox and re-add the projected mean (compute_projection.rs:94-121).
-        if (live) {
-            float o[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int ly = tr / vs, lx = (tc + k) / hs;
-                float r = yd[k];
-                if (cx0 + lx < (int)ch.rw && cy0 + ly < (int)ch.rh) {
-                    const float pr = blk[((ly >> 3) * nbx + (lx >> 3)) * BSTR + (ly & 7) * 9 + (lx & 7)];
-                    r = ch.resample ? (yd[k] - mv[ly][lx]) + pr : pr;
-                }
-                o[k] = r;
+                for (int k = 0; k < 8; k++) bp[j * 9 + k] = v[k];
+            } else {
+                float *row = ch.prob + (size_t)(by * 8 + j) * rw + bx * 8;
+                const float ap = ch.alpha_p;
+                *reinterpret_cast<float4 *>(row) = make_float4(ap * v[0], ap * v[1], ap * v[2], ap * v[3]);
+                *reinterpret_cast<float4 *>(row + 4) = make_float4(ap * v[4], ap * v[5], ap * v[6], ap * v[7]);
             }
-            *reinterpret_cast<float4 *>(a.xnew[c] + (size_t)qy * W + qx) = make_float4(o[0], o[1], o[2], o[3]);
         }
         __syncthreads();
+    }
+
+    // Unbox and re-add the projected mean (compute_projection.rs:94-121).
+    if (live) {
+        float o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int ly = tr / VS, lx = (tc + k) / HS;
+            float r = yd[k];
+            if (cx0 + lx < rw && cy0 + ly < rh) {
+                const float pr = sh.blk[((ly >> 3) * NBX + (lx >> 3)) * BSTR + (ly & 7) * 9 + (lx & 7)];
+                r = RES ? (yd[k] - sh.mv[ly][lx]) + pr : pr;
+            }
+            o[k] = r;
+        }
+        *reinterpret_cast<float4 *>(xnew + (size_t)(y0 + tr) * a.W + x0 + tc) = make_float4(o[0], o[1], o[2], o[3]);
+    }
+    __syncthreads();
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void k_project(const IterArgs a)
+{
+    __shared__ ProjShared sh;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int W = a.W, H = a.H;
+    const int tr = tid >> 4, tc = (tid & 15) * 4;
+    const int qy = y0 + tr, qx = x0 + tc;
+    const bool live = qy < H && qx < W;
+    const float f = a.factor, step = a.step;
+    const size_t p = live ? (size_t)qy * W + qx : 0;
+
+    // Every channel's loads first, so they are all in flight together.
+    float4 xv[C], pv[C], gv[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        xv[c] = *reinterpret_cast<const float4 *>(a.x[c] + p);
+        pv[c] = *reinterpret_cast<const float4 *>(a.xp[c] + p);
+        gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
+    }
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const ChanDev &ch = a.ch[c];
+        const float norm = a.norms[c];
+        const double rn = a.rnorm[c];
+        // Descent step on the extrapolated image (compute_step.rs:61-65).
+        const float xs[4] = {xv[c].x, xv[c].y, xv[c].z, xv[c].w};
+        const float ps[4] = {pv[c].x, pv[c].y, pv[c].z, pv[c].w};
+        const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
+        float yd[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float y = fista(xs[k], ps[k], f);
+            if (norm != 0.0f) y -= step * qdiv(gs[k], rn);
+            yd[k] = y;
+        }
+        switch (ch.mode) {
+        case PM_FULL: project_channel<1, 1, false>(a, ch, a.xnew[c], sh, yd, tr, tc, x0, y0, live); break;
+        case PM_SUB11: project_channel<1, 1, true>(a, ch, a.xnew[c], sh, yd, tr, tc, x0, y0, live); break;
+        case PM_SUB22: project_channel<2, 2, true>(a, ch, a.xnew[c], sh, yd, tr, tc, x0, y0, live); break;
+        case PM_SUB21: project_channel<2, 1, true>(a, ch, a.xnew[c], sh, yd, tr, tc, x0, y0, live); break;
+        default: project_channel<1, 2, true>(a, ch, a.xnew[c], sh, yd, tr, tc, x0, y0, live); break;
+        }
     }
 }
 
@@ -636,12 +735,13 @@ __global__ __launch_bounds__(NT) void k_init_upsample(const ChanDev ch, const fl
     xp[i] = v;
 }
 
-// cos = c * q (coef.rs:94-104).
+// cos = c * q (coef.rs:94-104); the prob term of the first iteration is 0 (cos == c q).
 __global__ __launch_bounds__(NT) void k_init_cos(const ChanDev ch)
 {
     const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
     if (i >= (size_t)ch.bw * ch.bh * 64) return;
     ch.cos[i] = (float)ch.coef[i] * ch.quant[i & 63];
+    ch.prob[i] = 0.0f;
 }
 
 struct DeviceGuard {
@@ -668,20 +768,23 @@ struct artefact_hip_plan {
     size_t M[3] = {0, 0, 0};
     float weight = 0.0f, pweight[3] = {0, 0, 0};
     bool tgv = true;
+    int rows = 8;  // k_gradient rows per wave
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
     ChanDev ch[3];
     float *buf[2] = {nullptr, nullptr};  // each C*N: x and x_prev, swapped every iteration
     float *g = nullptr;                  // C*N
     float *cos = nullptr;                // sum M_c
+    float *prob = nullptr;               // sum M_c
     int16_t *coef = nullptr;             // sum M_c
-    float *tables = nullptr;             // [3][2][64]: quant, quant^2
+    float *quant = nullptr;              // [3][64]
+    double *rqq = nullptr;               // [3][64]
     double *partials = nullptr;
-    unsigned int *counter = nullptr;
     float *norms = nullptr;
+    double *rnorm = nullptr;
     double *norm_sq = nullptr;
     std::vector<std::vector<float>> init_override;
-    dim3 grid;
+    dim3 grid_g, grid_p;
     int cur = 0;            // buf[cur] holds x
     bool initialized = false;
     bool use_graph = true;
@@ -696,6 +799,12 @@ int check_inputs(const artefact_params *p, const artefact_channel *ch)
     if (p->nchannel < 1 || p->nchannel > 3) return ARTEFACT_E_ARGUMENT;
     if (p->mode != ARTEFACT_MODE_TGV && p->mode != ARTEFACT_MODE_TV) return ARTEFACT_E_ARGUMENT;
     if (p->max_w < 8 || p->max_h < 8 || p->max_w % 8 || p->max_h % 8) return ARTEFACT_E_GEOMETRY;
+    // int16 coefficients, u16 tables and a bounded grid keep every iterate far below
+    // 2^60 in magnitude, so every quotient the kernels form is finite.
+    if ((uint64_t)p->max_w * p->max_h > (1ull << 29)) return ARTEFACT_E_GEOMETRY;
+    if (!(p->weight >= 0.0f && p->weight < 1e6f)) return ARTEFACT_E_ARGUMENT;
+    for (int c = 0; c < 3; c++)
+        if (!(p->pweight[c] >= 0.0f && p->pweight[c] < 1e6f)) return ARTEFACT_E_ARGUMENT;
     for (uint32_t c = 0; c < p->nchannel; c++) {
         const artefact_channel &k = ch[c];
         if (!k.coefs || !k.quant) return ARTEFACT_E_ARGUMENT;
@@ -715,27 +824,25 @@ void free_plan(artefact_hip_plan *pl)
     if (!pl) return;
     DeviceGuard guard(pl->device);
     if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
-    (void)hipFree(pl->buf[0]);
-    (void)hipFree(pl->buf[1]);
-    (void)hipFree(pl->g);
-    (void)hipFree(pl->cos);
-    (void)hipFree(pl->coef);
-    (void)hipFree(pl->tables);
-    (void)hipFree(pl->partials);
-    (void)hipFree(pl->counter);
-    (void)hipFree(pl->norms);
-    (void)hipFree(pl->norm_sq);
+    void *bufs[] = {pl->buf[0], pl->buf[1], pl->g,        pl->cos,     pl->prob,  pl->coef, pl->quant,
+                    pl->rqq,    pl->partials, pl->norms, pl->rnorm, pl->norm_sq};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);
     delete pl;
 }
 
-IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor)
+IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool write_cos)
 {
     IterArgs a;
     memset(&a, 0, sizeof(a));
     a.W = pl->W;
     a.H = pl->H;
     a.C = pl->C;
+    a.write_cos = write_cos ? 1u : 0u;
+    size_t mtot = 0;
+    for (uint32_t c = 0; c < pl->C; c++) mtot += pl->M[c];
+    a.prob_bytes = (uint32_t)(mtot * sizeof(float));
     a.factor = factor;
     a.step = pl->step;
     a.alpha_tv = pl->alpha_tv;
@@ -749,25 +856,83 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor)
         a.ch[c] = pl->ch[c < pl->C ? c : 0];
     }
     a.partials = pl->partials;
-    a.counter = pl->counter;
     a.norms = pl->norms;
+    a.rnorm = pl->rnorm;
     a.norm_sq = pl->norm_sq;
     return a;
+}
+
+template <int C, bool TGV>
+void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a)
+{
+    switch (pl->rows) {
+    case 4: hipLaunchKernelGGL((k_gradient<C, TGV, 4>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
+    case 8: hipLaunchKernelGGL((k_gradient<C, TGV, 8>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
+    case 12: hipLaunchKernelGGL((k_gradient<C, TGV, 12>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
+    default: hipLaunchKernelGGL((k_gradient<C, TGV, 16>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
+    }
+    const unsigned int nwg = pl->grid_g.x * pl->grid_g.y;
+    hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, pl->stream, (const double *)pl->partials, nwg,
+                       pl->norms, pl->rnorm, pl->norm_sq);
+}
+
+// Gradient sweep + norm: k_gradient then k_finalize.
+void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a)
+{
+    switch (pl->C * 2 + (pl->tgv ? 1 : 0)) {
+    case 2: launch_gradient_c<1, false>(pl, a); break;
+    case 3: launch_gradient_c<1, true>(pl, a); break;
+    case 4: launch_gradient_c<2, false>(pl, a); break;
+    case 5: launch_gradient_c<2, true>(pl, a); break;
+    case 6: launch_gradient_c<3, false>(pl, a); break;
+    default: launch_gradient_c<3, true>(pl, a); break;
+    }
+}
+
+void launch_project(const artefact_hip_plan *pl, const IterArgs &a)
+{
+    if (pl->C == 1)
+        hipLaunchKernelGGL(k_project<1>, pl->grid_p, dim3(NT), 0, pl->stream, a);
+    else if (pl->C == 2)
+        hipLaunchKernelGGL(k_project<2>, pl->grid_p, dim3(NT), 0, pl->stream, a);
+    else
+        hipLaunchKernelGGL(k_project<3>, pl->grid_p, dim3(NT), 0, pl->stream, a);
 }
 
 int launch_iterations(artefact_hip_plan *pl)
 {
     int cur = pl->cur;
     for (uint32_t it = 0; it < pl->iterations; it++) {
-        const IterArgs a = make_args(pl, cur, pl->factors[it]);
-        if (pl->tgv)
-            hipLaunchKernelGGL(k_gradient<true>, pl->grid, dim3(NT), 0, pl->stream, a);
-        else
-            hipLaunchKernelGGL(k_gradient<false>, pl->grid, dim3(NT), 0, pl->stream, a);
-        hipLaunchKernelGGL(k_project, pl->grid, dim3(NT), 0, pl->stream, a);
+        const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == pl->iterations);
+        launch_gradient(pl, a);
+        launch_project(pl, a);
         cur = 1 - cur;
     }
     return hipGetLastError() == hipSuccess ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+// k_gradient rows per wave.  A wave reads R + 4 rows and evaluates R + 2 rows of sources
+// for R output rows, so taller strips waste less; but the grid should not spill a
+// partial extra round over the ~4 resident waves per SIMD (1024 SIMDs).  Pick the R with
+// the smallest rounds * (R + 3.5).  An override (ARTEFACT_KG_ROWS) is honoured.
+int pick_rows(uint32_t W, uint32_t H)
+{
+    if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
+        const int r = atoi(e);
+        if (r == 4 || r == 8 || r == 12 || r == 16) return r;
+    }
+    const uint64_t cols = (W + KG_COLS - 1) / KG_COLS, slots = 4096;
+    int best = 4;
+    double best_cost = 1e300;
+    for (int r : {4, 8, 12, 16}) {
+        const uint64_t waves = cols * ((H + r - 1) / r);
+        const double cost = (double)((waves + slots - 1) / slots) * (r + 3.5);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = r;
+        }
+    }
+    return best;
 }
 
 }  // namespace
@@ -842,39 +1007,45 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         pl->M[c] = (size_t)chs[c].rounded_w * chs[c].rounded_h;
         mtot += pl->M[c];
     }
-    pl->grid = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
-    const size_t nwg = (size_t)pl->grid.x * pl->grid.y;
+    pl->rows = pick_rows(pl->W, pl->H);
+    pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * pl->rows - 1) / (WAVES * pl->rows));
+    pl->grid_p = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
+    const size_t nwg = (size_t)pl->grid_g.x * pl->grid_g.y;
     bool ok = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&pl->buf[0], pl->C * pl->N * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->buf[1], pl->C * pl->N * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->g, pl->C * pl->N * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->cos, mtot * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&pl->prob, mtot * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->coef, mtot * sizeof(int16_t)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->tables, 3 * 128 * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&pl->quant, 3 * 64 * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&pl->rqq, 3 * 64 * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&pl->partials, nwg * 6 * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->counter, sizeof(unsigned int)) == hipSuccess;
     ok = ok && hipMalloc(&pl->norms, 3 * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&pl->rnorm, 3 * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&pl->norm_sq, 3 * sizeof(double)) == hipSuccess;
     if (!ok) {
         free_plan(pl);
         return ARTEFACT_E_HIP;
     }
-    std::vector<float> tables(3 * 128, 1.0f);
+    std::vector<float> quant(3 * 64, 1.0f);
+    std::vector<double> rqq(3 * 64, 1.0);
     size_t moff = 0;
     for (uint32_t c = 0; c < pl->C; c++) {
         const artefact_channel &k = chs[c];
         for (int j = 0; j < 64; j++) {
             const float q = (float)k.quant[j];
-            tables[c * 128 + j] = q;
-            tables[c * 128 + 64 + j] = q * q;
+            quant[c * 64 + j] = q;
+            rqq[c * 64 + j] = 1.0 / (double)(q * q);  // q*q rounded to f32 first, as compute_step_prob.rs:29
         }
         ok = ok && hipMemcpyAsync(pl->coef + moff, k.coefs, pl->M[c] * sizeof(int16_t), hipMemcpyHostToDevice,
                                   pl->stream) == hipSuccess;
         ChanDev &d = pl->ch[c];
         d.coef = pl->coef + moff;
         d.cos = pl->cos + moff;
-        d.quant = pl->tables + c * 128;
-        d.qq = pl->tables + c * 128 + 64;
+        d.prob = pl->prob + moff;
+        d.quant = pl->quant + c * 64;
+        d.rqq = pl->rqq + c * 64;
         d.rw = k.rounded_w;
         d.rh = k.rounded_h;
         d.bw = k.block_w;
@@ -882,14 +1053,21 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         d.hs = k.h_samp;
         d.vs = k.v_samp;
         d.resample = (k.rounded_w != pl->W || k.rounded_h != pl->H) ? 1u : 0u;
+        if (!d.resample) d.mode = PM_FULL;
+        else if (d.hs == 1 && d.vs == 1) d.mode = PM_SUB11;
+        else if (d.hs == 2 && d.vs == 2) d.mode = PM_SUB22;
+        else if (d.hs == 2) d.mode = PM_SUB21;
+        else d.mode = PM_SUB12;
         d.alpha_p = pl->pweight[c] != 0.0f ? pl->pweight[c] * 2.0f * 255.0f * sqrtf(2.0f) : 0.0f;
         moff += pl->M[c];
     }
     for (uint32_t c = pl->C; c < 3; c++) pl->ch[c] = pl->ch[0];
-    ok = ok && hipMemcpyAsync(pl->tables, tables.data(), tables.size() * sizeof(float), hipMemcpyHostToDevice,
+    ok = ok && hipMemcpyAsync(pl->quant, quant.data(), quant.size() * sizeof(float), hipMemcpyHostToDevice,
                               pl->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(pl->counter, 0, sizeof(unsigned int), pl->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(pl->rqq, rqq.data(), rqq.size() * sizeof(double), hipMemcpyHostToDevice,
+                              pl->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(pl->norms, 0, 3 * sizeof(float), pl->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(pl->rnorm, 0, 3 * sizeof(double), pl->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(pl->norm_sq, 0, 3 * sizeof(double), pl->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(pl->stream) == hipSuccess;
     if (!ok) {
@@ -960,8 +1138,8 @@ int artefact_hip_plan_iterate(artefact_hip_plan *pl)
         if (rc) return rc;
     }
     if (pl->iterations % 2) pl->cur = 1 - pl->cur;
-    // The iterate leaves x in buf[cur]; a second iterate would restart the FISTA
-    // sequence from the current image, which the reference never does.
+    // The loop consumed the start image; a new solve needs a new init (the reference
+    // never restarts the FISTA sequence from an iterate).
     pl->initialized = false;
     return ARTEFACT_OK;
 }
@@ -1008,11 +1186,7 @@ int artefact_hip_plan_first_gradient(artefact_hip_plan *pl, float *const *out)
     if (!pl || !out) return ARTEFACT_E_ARGUMENT;
     if (!pl->initialized) return ARTEFACT_E_STATE;
     DeviceGuard guard(pl->device);
-    const IterArgs a = make_args(pl, pl->cur, 0.0f);
-    if (pl->tgv)
-        hipLaunchKernelGGL(k_gradient<true>, pl->grid, dim3(NT), 0, pl->stream, a);
-    else
-        hipLaunchKernelGGL(k_gradient<false>, pl->grid, dim3(NT), 0, pl->stream, a);
+    launch_gradient(pl, make_args(pl, pl->cur, 0.0f, false));
     bool ok = hipGetLastError() == hipSuccess;
     for (uint32_t c = 0; c < pl->C; c++) {
         if (!out[c]) return ARTEFACT_E_ARGUMENT;
@@ -1036,13 +1210,10 @@ int artefact_hip_plan_profile(artefact_hip_plan *pl, float *avg_ms)
     if (ok && n) {
         ok = hipEventRecord(ev[0], pl->stream) == hipSuccess;
         for (uint32_t it = 0; ok && it < n; it++) {
-            const IterArgs a = make_args(pl, cur, pl->factors[it]);
-            if (pl->tgv)
-                hipLaunchKernelGGL(k_gradient<true>, pl->grid, dim3(NT), 0, pl->stream, a);
-            else
-                hipLaunchKernelGGL(k_gradient<false>, pl->grid, dim3(NT), 0, pl->stream, a);
+            const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == n);
+            launch_gradient(pl, a);
             ok = ok && hipEventRecord(ev[2 * it + 1], pl->stream) == hipSuccess;
-            hipLaunchKernelGGL(k_project, pl->grid, dim3(NT), 0, pl->stream, a);
+            launch_project(pl, a);
             ok = ok && hipEventRecord(ev[2 * it + 2], pl->stream) == hipSuccess;
             cur = 1 - cur;
         }
