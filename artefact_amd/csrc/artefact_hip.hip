@@ -64,6 +64,23 @@ constexpr int TW = 64;          // k_project tile width (full-resolution pixels)
 constexpr int TH = 16;          // k_project tile height
 constexpr int BSTR = 72;        // LDS floats per 8x8 block: 8 rows of 9 (bank spread)
 constexpr int MAXBLK = 16;      // 8x8 blocks per k_project tile at most (2 x 8)
+constexpr int NXCD = 8;         // accelerator complex dies on gfx950; workgroups are dealt round-robin
+
+// Logical tile of this workgroup such that the tiles one XCD runs form a contiguous band
+// of rows: the dispatcher sends workgroup i to XCD i % NXCD, and every XCD has its own L2,
+// so neighbouring strips (which share halo rows and cache lines) should share an XCD.
+__device__ __forceinline__ void xcd_tile(unsigned int &tx, unsigned int &ty)
+{
+    const unsigned int n = gridDim.x * gridDim.y;
+    const unsigned int h = blockIdx.y * gridDim.x + blockIdx.x;
+    const unsigned int xcd = h % NXCD, j = h / NXCD;
+    // XCD k receives ceil((n - k) / NXCD) workgroups; those before it hold the earlier tiles.
+    const unsigned int q = n / NXCD, r = n % NXCD;
+    const unsigned int base = xcd * q + (xcd < r ? xcd : r);
+    const unsigned int l = base + j;
+    ty = l / gridDim.x;
+    tx = l % gridDim.x;
+}
 
 // Ooura shrtdct 8-point constants, backend/artefact-lib/utils/dct.rs:20-27.
 constexpr float C8_1R = 0.49039264020161522456f;
@@ -214,6 +231,8 @@ __device__ __forceinline__ DD dd_shfl_xor(DD a, int m)
 }
 
 enum ProjMode : uint32_t { PM_FULL = 0, PM_SUB11 = 1, PM_SUB22 = 2, PM_SUB21 = 3, PM_SUB12 = 4 };
+// Channel layouts with a fused projection kernel; anything else runs channel by channel.
+enum Layout : int { LAYOUT_GENERIC = 0, LAYOUT_Y = 1, LAYOUT_444 = 2, LAYOUT_420 = 3 };
 
 struct ChanDev {
     const int16_t *coef;  // M_c, block order
@@ -270,8 +289,10 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = a.W, H = a.H;
-    const int x = (int)blockIdx.x * KG_COLS - 2 + lane;
-    const int r0 = ((int)blockIdx.y * WAVES + wave) * R;
+    unsigned int tx, ty;
+    xcd_tile(tx, ty);
+    const int x = (int)tx * KG_COLS - 2 + lane;
+    const int r0 = ((int)ty * WAVES + wave) * R;
     const bool xin = x >= 0 && x < W;
     const bool out_lane = lane >= 2 && lane < 62 && xin;
     const bool xl = x > 0;
@@ -447,7 +468,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned int wg = blockIdx.y * gridDim.x + blockIdx.x;
+        const unsigned int wg = ty * gridDim.x + tx;
 #pragma unroll
         for (int c = 0; c < C; c++) {
             DD v = red[0][c];
@@ -689,6 +710,273 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
     }
 }
 
+// Compile-time geometry of a channel inside a k_project tile.
+template <int MODE> struct ModeGeo;
+template <> struct ModeGeo<PM_FULL> { static constexpr int HS = 1, VS = 1; static constexpr bool RES = false; };
+template <> struct [[maybe_unused]] ModeGeo<PM_SUB11> { static constexpr int HS = 1, VS = 1; static constexpr bool RES = true; };
+template <> struct ModeGeo<PM_SUB22> { static constexpr int HS = 2, VS = 2; static constexpr bool RES = true; };
+template <> struct ModeGeo<PM_SUB21> { static constexpr int HS = 2, VS = 1; static constexpr bool RES = true; };
+template <> struct ModeGeo<PM_SUB12> { static constexpr int HS = 1, VS = 2; static constexpr bool RES = true; };
+template <int MODE> constexpr int mode_nbx() { return TW / ModeGeo<MODE>::HS / 8; }
+template <int MODE> constexpr int mode_nb() { return mode_nbx<MODE>() * (TH / ModeGeo<MODE>::VS / 8); }
+
+template <int C, int M0, int M1, int M2> struct FusedLayout {
+    static constexpr int NB0 = mode_nb<M0>();
+    static constexpr int NB1 = C > 1 ? mode_nb<M1>() : 0;
+    static constexpr int NB2 = C > 2 ? mode_nb<M2>() : 0;
+    static constexpr int TOTB = NB0 + NB1 + NB2;
+    // LDS slots for the resampled channels' full-resolution tile and box means.
+    static constexpr int R0 = ModeGeo<M0>::RES ? 1 : 0;
+    static constexpr int R1 = C > 1 && ModeGeo<M1>::RES ? 1 : 0;
+    static constexpr int R2 = C > 2 && ModeGeo<M2>::RES ? 1 : 0;
+    static constexpr int NRES = R0 + R1 + R2 > 0 ? R0 + R1 + R2 : 1;
+    static constexpr int SLOT1 = R0, SLOT2 = R0 + R1;
+};
+
+template <int C, int M0, int M1, int M2> struct FusedShared {
+    float blk[FusedLayout<C, M0, M1, M2>::TOTB * BSTR];   // projection blocks of every channel
+    float pbk[FusedLayout<C, M0, M1, M2>::TOTB * BSTR];   // prob-term blocks
+    float yv[FusedLayout<C, M0, M1, M2>::NRES][TH][TW + 1];  // descended values (resampled channels)
+    float mv[FusedLayout<C, M0, M1, M2>::NRES][TH][TW + 1];  // their box means
+};
+
+// Stage one channel's descended tile into LDS: straight into 8x8 blocks at full
+// resolution, or into yv for the box means (compute_projection.rs:18-61).
+template <int SLOT, int MODE, int BOFF, typename SH>
+__device__ __forceinline__ void fused_stage(SH &sh, const float yd[4], int tr, int tc)
+{
+    if (ModeGeo<MODE>::RES) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) sh.yv[SLOT][tr][tc + k] = yd[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            sh.blk[(BOFF + (tr >> 3) * 8 + ((tc + k) >> 3)) * BSTR + (tr & 7) * 9 + ((tc + k) & 7)] = yd[k];
+    }
+}
+
+template <int SLOT, int MODE, int BOFF, typename SH>
+__device__ __forceinline__ void fused_means(SH &sh, const ChanDev &ch, int x0, int y0)
+{
+    if (!ModeGeo<MODE>::RES) return;
+    constexpr int HS = ModeGeo<MODE>::HS, VS = ModeGeo<MODE>::VS;
+    constexpr int CW = TW / HS, CT = TH / VS, NBX = CW / 8;
+    const int cx0 = x0 / HS, cy0 = y0 / VS;
+    for (int i = threadIdx.x; i < CW * CT; i += NT) {
+        const int ly = i / CW, lx = i % CW;
+        if (cx0 + lx < (int)ch.rw && cy0 + ly < (int)ch.rh) {
+            float m = 0.0f;
+#pragma unroll
+            for (int sy = 0; sy < VS; sy++)
+#pragma unroll
+                for (int sx = 0; sx < HS; sx++) m += sh.yv[SLOT][ly * VS + sy][lx * HS + sx];
+            m /= (float)(HS * VS);
+            sh.mv[SLOT][ly][lx] = m;
+            sh.blk[(BOFF + (ly >> 3) * NBX + (lx >> 3)) * BSTR + (ly & 7) * 9 + (lx & 7)] = m;
+        }
+    }
+}
+
+template <int SLOT, int MODE, int BOFF, typename SH>
+__device__ __forceinline__ void fused_emit(SH &sh, const ChanDev &ch, float *xnew, int W, const float yd[4], int tr,
+                                           int tc, int x0, int y0, bool live)
+{
+    constexpr int HS = ModeGeo<MODE>::HS, VS = ModeGeo<MODE>::VS, NBX = mode_nbx<MODE>();
+    if (!live) return;
+    const int cx0 = x0 / HS, cy0 = y0 / VS;
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int ly = tr / VS, lx = (tc + k) / HS;
+        float r = yd[k];
+        if (cx0 + lx < (int)ch.rw && cy0 + ly < (int)ch.rh) {
+            const float pr = sh.blk[(BOFF + (ly >> 3) * NBX + (lx >> 3)) * BSTR + (ly & 7) * 9 + (lx & 7)];
+            r = ModeGeo<MODE>::RES ? (yd[k] - sh.mv[SLOT][ly][lx]) + pr : pr;
+        }
+        o[k] = r;
+    }
+    *reinterpret_cast<float4 *>(xnew + (size_t)(y0 + tr) * W + x0 + tc) = make_float4(o[0], o[1], o[2], o[3]);
+}
+
+// Which channel block b of the tile belongs to, and its block coordinates in that
+// channel's component plane.
+template <int C, int M0, int M1, int M2>
+__device__ __forceinline__ int fused_block(int b, int x0, int y0, int &bx, int &by)
+{
+    using L = FusedLayout<C, M0, M1, M2>;
+    if (b < L::NB0) {
+        bx = x0 / ModeGeo<M0>::HS / 8 + b % mode_nbx<M0>();
+        by = y0 / ModeGeo<M0>::VS / 8 + b / mode_nbx<M0>();
+        return 0;
+    }
+    if (C > 1 && b < L::NB0 + L::NB1) {
+        const int lb = b - L::NB0;
+        bx = x0 / ModeGeo<M1>::HS / 8 + lb % mode_nbx<M1>();
+        by = y0 / ModeGeo<M1>::VS / 8 + lb / mode_nbx<M1>();
+        return 1;
+    }
+    const int lb = b - L::NB0 - L::NB1;
+    bx = x0 / ModeGeo<M2>::HS / 8 + lb % mode_nbx<M2>();
+    by = y0 / ModeGeo<M2>::VS / 8 + lb / mode_nbx<M2>();
+    return 2;
+}
+
+// k_project for the common layouts (channel modes known at compile time): every
+// channel's blocks go through one DCT pass and one double IDCT pass together.
+template <int C, int M0, int M1, int M2>
+__global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
+{
+    using L = FusedLayout<C, M0, M1, M2>;
+    constexpr int TOTB = L::TOTB;
+    __shared__ FusedShared<C, M0, M1, M2> sh;
+    const int tid = threadIdx.x;
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    const int W = a.W, H = a.H;
+    const int tr = tid >> 4, tc = (tid & 15) * 4;
+    const int qy = y0 + tr, qx = x0 + tc;
+    const bool live = qy < H && qx < W;
+    const float f = a.factor, step = a.step;
+    const size_t p = live ? (size_t)qy * W + qx : 0;
+
+    float4 xv[C], pv[C], gv[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        xv[c] = *reinterpret_cast<const float4 *>(a.x[c] + p);
+        pv[c] = *reinterpret_cast<const float4 *>(a.xp[c] + p);
+        gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
+    }
+    // Descent step on the extrapolated image (compute_step.rs:61-65).
+    float yd[C][4];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const float norm = a.norms[c];
+        const double rn = a.rnorm[c];
+        const float xs[4] = {xv[c].x, xv[c].y, xv[c].z, xv[c].w};
+        const float ps[4] = {pv[c].x, pv[c].y, pv[c].z, pv[c].w};
+        const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float y = fista(xs[k], ps[k], f);
+            if (norm != 0.0f) y -= step * qdiv(gs[k], rn);
+            yd[c][k] = y;
+        }
+    }
+    fused_stage<0, M0, 0>(sh, yd[0], tr, tc);
+    if constexpr (C > 1) fused_stage<L::SLOT1, M1, L::NB0>(sh, yd[1], tr, tc);
+    if constexpr (C > 2) fused_stage<L::SLOT2, M2, L::NB0 + L::NB1>(sh, yd[2], tr, tc);
+    __syncthreads();
+    if (ModeGeo<M0>::RES || (C > 1 && ModeGeo<M1>::RES) || (C > 2 && ModeGeo<M2>::RES)) {
+        fused_means<0, M0, 0>(sh, a.ch[0], x0, y0);
+        if constexpr (C > 1) fused_means<L::SLOT1, M1, L::NB0>(sh, a.ch[1], x0, y0);
+        if constexpr (C > 2) fused_means<L::SLOT2, M2, L::NB0 + L::NB1>(sh, a.ch[2], x0, y0);
+        __syncthreads();
+    }
+
+    // Forward DCT, column pass (utils/dct.rs:100-134).
+    for (int jb = tid; jb < TOTB * 8; jb += NT) {
+        const int b = jb >> 3, j = jb & 7;
+        int bx, by;
+        const int c = fused_block<C, M0, M1, M2>(b, x0, y0, bx, by);
+        const ChanDev &ch = a.ch[c];
+        if (bx < (int)ch.bw && by < (int)ch.bh) {
+            float *bp = sh.blk + b * BSTR;
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bp[k * 9 + j];
+            dct8(v);
+#pragma unroll
+            for (int k = 0; k < 8; k++) bp[k * 9 + j] = v[k];
+        }
+    }
+    __syncthreads();
+    // Row pass, clamp to [(c - 1/2) q, (c + 1/2) q] (compute_projection.rs:63-82) and the
+    // prob difference (cos - c q) / q^2 (compute_step_prob.rs:24-30).
+    for (int jb = tid; jb < TOTB * 8; jb += NT) {
+        const int b = jb >> 3, j = jb & 7;
+        int bx, by;
+        const int c = fused_block<C, M0, M1, M2>(b, x0, y0, bx, by);
+        const ChanDev &ch = a.ch[c];
+        if (bx < (int)ch.bw && by < (int)ch.bh) {
+            float *bp = sh.blk + b * BSTR;
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bp[j * 9 + k];
+            dct8(v);
+            const size_t base = ((size_t)by * ch.bw + bx) * 64 + j * 8;
+            const short4 c03 = *reinterpret_cast<const short4 *>(ch.coef + base);
+            const short4 c47 = *reinterpret_cast<const short4 *>(ch.coef + base + 4);
+            const float cf[8] = {(float)c03.x, (float)c03.y, (float)c03.z, (float)c03.w,
+                                 (float)c47.x, (float)c47.y, (float)c47.z, (float)c47.w};
+            float o[8], d[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float q = ch.quant[j * 8 + k];
+                o[k] = clamp_ref(v[k], (cf[k] - 0.5f) * q, (cf[k] + 0.5f) * q);
+                d[k] = qdiv(o[k] - cf[k] * q, ch.rqq[j * 8 + k]);
+            }
+            if (a.write_cos) {
+                *reinterpret_cast<float4 *>(ch.cos + base) = make_float4(o[0], o[1], o[2], o[3]);
+                *reinterpret_cast<float4 *>(ch.cos + base + 4) = make_float4(o[4], o[5], o[6], o[7]);
+            }
+            float *pp = sh.pbk + b * BSTR;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                bp[j * 9 + k] = o[k];
+                pp[j * 9 + k] = d[k];
+            }
+        }
+    }
+    __syncthreads();
+    // Two IDCTs (utils/dct.rs:30-98): the projected blocks and the prob blocks.
+    for (int jb = tid; jb < 2 * TOTB * 8; jb += NT) {
+        const int half = jb >= TOTB * 8, jr = jb - half * TOTB * 8;
+        const int b = jr >> 3, j = jr & 7;
+        int bx, by;
+        const int c = fused_block<C, M0, M1, M2>(b, x0, y0, bx, by);
+        const ChanDev &ch = a.ch[c];
+        if (bx < (int)ch.bw && by < (int)ch.bh && (!half || ch.alpha_p != 0.0f)) {
+            float *bp = (half ? sh.pbk : sh.blk) + b * BSTR;
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bp[k * 9 + j];
+            idct8(v);
+#pragma unroll
+            for (int k = 0; k < 8; k++) bp[k * 9 + j] = v[k];
+        }
+    }
+    __syncthreads();
+    for (int jb = tid; jb < 2 * TOTB * 8; jb += NT) {
+        const int half = jb >= TOTB * 8, jr = jb - half * TOTB * 8;
+        const int b = jr >> 3, j = jr & 7;
+        int bx, by;
+        const int c = fused_block<C, M0, M1, M2>(b, x0, y0, bx, by);
+        const ChanDev &ch = a.ch[c];
+        if (bx < (int)ch.bw && by < (int)ch.bh && (!half || ch.alpha_p != 0.0f)) {
+            float *bp = (half ? sh.pbk : sh.blk) + b * BSTR;
+            float v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = bp[j * 9 + k];
+            idct8(v);
+            if (!half) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) bp[j * 9 + k] = v[k];
+            } else {
+                float *row = ch.prob + (size_t)(by * 8 + j) * ch.rw + bx * 8;
+                const float ap = ch.alpha_p;
+                *reinterpret_cast<float4 *>(row) = make_float4(ap * v[0], ap * v[1], ap * v[2], ap * v[3]);
+                *reinterpret_cast<float4 *>(row + 4) = make_float4(ap * v[4], ap * v[5], ap * v[6], ap * v[7]);
+            }
+        }
+    }
+    __syncthreads();
+    // Unbox and re-add the projected mean (compute_projection.rs:94-121).
+    fused_emit<0, M0, 0>(sh, a.ch[0], a.xnew[0], W, yd[0], tr, tc, x0, y0, live);
+    if constexpr (C > 1)
+        fused_emit<L::SLOT1, M1, L::NB0>(sh, a.ch[1], a.xnew[1], W, yd[1], tr, tc, x0, y0, live);
+    if constexpr (C > 2)
+        fused_emit<L::SLOT2, M2, L::NB0 + L::NB1>(sh, a.ch[2], a.xnew[2], W, yd[2], tr, tc, x0, y0, live);
+}
+
 // Start image, step 1: dequantise and IDCT every block into the component plane
 // (pipeline_simd_8/coef.rs:88-118: all blocks, then one unboxing).
 __global__ __launch_bounds__(NT) void k_init_blocks(const ChanDev ch, float *plane)
@@ -769,6 +1057,7 @@ struct artefact_hip_plan {
     float weight = 0.0f, pweight[3] = {0, 0, 0};
     bool tgv = true;
     int rows = 8;  // k_gradient rows per wave
+    int layout = 0;  // LAYOUT_*: which k_project variant runs
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
     ChanDev ch[3];
@@ -891,6 +1180,18 @@ void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a)
 
 void launch_project(const artefact_hip_plan *pl, const IterArgs &a)
 {
+    switch (pl->layout) {
+    case LAYOUT_Y:
+        hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, pl->stream, a);
+        return;
+    case LAYOUT_444:
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, pl->stream, a);
+        return;
+    case LAYOUT_420:
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_SUB22, PM_SUB22>), pl->grid_p, dim3(NT), 0, pl->stream, a);
+        return;
+    default: break;
+    }
     if (pl->C == 1)
         hipLaunchKernelGGL(k_project<1>, pl->grid_p, dim3(NT), 0, pl->stream, a);
     else if (pl->C == 2)
@@ -1062,6 +1363,14 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         moff += pl->M[c];
     }
     for (uint32_t c = pl->C; c < 3; c++) pl->ch[c] = pl->ch[0];
+    if (pl->C == 1 && pl->ch[0].mode == PM_FULL) pl->layout = LAYOUT_Y;
+    else if (pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_FULL && pl->ch[2].mode == PM_FULL)
+        pl->layout = LAYOUT_444;
+    else if (pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_SUB22 && pl->ch[2].mode == PM_SUB22)
+        pl->layout = LAYOUT_420;
+    else pl->layout = LAYOUT_GENERIC;
+    if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
+        if (atoi(e)) pl->layout = LAYOUT_GENERIC;
     ok = ok && hipMemcpyAsync(pl->quant, quant.data(), quant.size() * sizeof(float), hipMemcpyHostToDevice,
                               pl->stream) == hipSuccess;
     ok = ok && hipMemcpyAsync(pl->rqq, rqq.data(), rqq.size() * sizeof(double), hipMemcpyHostToDevice,

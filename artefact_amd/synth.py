@@ -122,6 +122,16 @@ def downsample2x2(p: np.ndarray) -> np.ndarray:
     return ((s + 2) // 4).astype(np.uint8)
 
 
+def downsample(p: np.ndarray, hs: int, vs: int) -> np.ndarray:
+    """Box average over hs x vs, rounded half up."""
+    if (hs, vs) == (2, 2):
+        return downsample2x2(p)
+    q = p.astype(np.int64)
+    s = sum(q[i::vs, j::hs] for i in range(vs) for j in range(hs))
+    n = hs * vs
+    return ((s + n // 2) // n).astype(np.uint8)
+
+
 def _pad_to(p: np.ndarray, h: int, w: int) -> np.ndarray:
     return np.pad(p, ((0, h - p.shape[0]), (0, w - p.shape[1])), mode="edge")
 
@@ -147,7 +157,7 @@ def plane_to_coefficient(plane: np.ndarray, q: np.ndarray, h_samp: int, v_samp: 
 def encode(planes, subsampling: str = "420", quality: int = 75) -> list[Coefficient]:
     """Coefficients of an 8-bit image: one plane (grey) or (R, G, B).
 
-    ``subsampling`` is "444" or "420" for colour input.
+    ``subsampling`` is "444", "420", "422" or "440" for colour input.
     """
     if len(planes) == 1:
         p = planes[0]
@@ -160,14 +170,16 @@ def encode(planes, subsampling: str = "420", quality: int = 75) -> list[Coeffici
     if subsampling == "444":
         H, W = -(-h // 8) * 8, -(-w // 8) * 8
         return [plane_to_coefficient(_pad_to(a, H, W), t, 1, 1) for a, t in ((y, ql), (cb, qc), (cr, qc))]
-    if subsampling != "420":
-        raise ValueError("subsampling must be '444' or '420'")
-    H, W = -(-h // 16) * 16, -(-w // 16) * 16
+    factors = {"420": (2, 2), "422": (2, 1), "440": (1, 2)}
+    if subsampling not in factors:
+        raise ValueError("subsampling must be '444', '420', '422' or '440'")
+    hs, vs = factors[subsampling]
+    H, W = -(-h // (8 * vs)) * 8 * vs, -(-w // (8 * hs)) * 8 * hs
     y, cb, cr = (_pad_to(a, H, W) for a in (y, cb, cr))
     return [
         plane_to_coefficient(y, ql, 1, 1),
-        plane_to_coefficient(downsample2x2(cb), qc, 2, 2),
-        plane_to_coefficient(downsample2x2(cr), qc, 2, 2),
+        plane_to_coefficient(downsample(cb, hs, vs), qc, hs, vs),
+        plane_to_coefficient(downsample(cr, hs, vs), qc, hs, vs),
     ]
 
 
@@ -182,7 +194,7 @@ def workload(name: str, seed: int = 0) -> dict:
     ``tv_luma_512``   plain TV, luma only, 512x512, quality 50, 50 iterations.
     ``tgv_420_2048``  plain TGV, YCbCr 4:2:0, 2048x1536, quality 75, 100 iterations.
     ``tgv_444_4096``  4:4:4 4096x3072, quality 30, 100 iterations.
-    Small variants for tests: ``tgv_420_<w>x<h>``, ``tgv_444_<w>x<h>``, ``tv_luma_<w>x<h>``
+    Small variants for tests: ``tgv_<420|422|440|444>_<w>x<h>``, ``tv_luma_<w>x<h>``
     with quality 75 (TGV) / 50 (TV).
     """
     presets = {

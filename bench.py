@@ -37,6 +37,35 @@ def algorithmic_bytes(coefs, W, H):
     return sum(20 * N + 16 * c.rounded_px_count for c in coefs)
 
 
+def copy_calibration(nbytes=256 << 20, reps=4):
+    """Device-to-device copy of a known size (HIP runtime copy kernel), timed with HIP
+    events: the achievable streaming bandwidth on this box, and under rocprofv3 --pmc
+    the FETCH_SIZE / WRITE_SIZE reference that scales the kernels' counters."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)):
+        return None
+    ev0, ev1 = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.hipEventCreate(ctypes.byref(ev0))
+    hip.hipEventCreate(ctypes.byref(ev1))
+    hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3)  # warm-up, hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipEventRecord(ev0, None)
+    for _ in range(reps):
+        hip.hipMemcpyAsync(dst, src, ctypes.c_size_t(nbytes), 3, None)
+    hip.hipEventRecord(ev1, None)
+    hip.hipEventSynchronize(ev1)
+    ms = ctypes.c_float()
+    hip.hipEventElapsedTime(ctypes.byref(ms), ev0, ev1)
+    hip.hipEventDestroy(ev0)
+    hip.hipEventDestroy(ev1)
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return {"bytes_read_per_copy": nbytes, "copy_GBps": round(2 * nbytes * reps / (ms.value * 1e-3) / 1e9, 1)}
+
+
 def cpu_baseline(wl, iters):
     import oracle
 
@@ -105,6 +134,7 @@ def main() -> int:
     out = plan.download()
     finite = all(bool(np.isfinite(p).all()) for p in out)
     plan.close()
+    calib = copy_calibration()
 
     res = {
         "metric": "Mpixel*iterations/s (plain TGV 4:2:0 2048x1536, 100 iterations)",
@@ -136,6 +166,7 @@ def main() -> int:
             "implementation_bytes_per_iteration": impl_bytes,
             "k_gradient_ms": round(prof["k_gradient_ms"], 5),
             "k_project_ms": round(prof["k_project_ms"], 5),
+            "copy_calibration": calib,
         },
         "finite_output": finite,
     }

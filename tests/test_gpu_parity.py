@@ -139,3 +139,33 @@ def test_requantises_at_full_size():
         q = k.quant_table.astype(np.float64)
         dev = np.abs(coef / q - k.dct_coefs.reshape(-1, 64))
         assert dev.max() <= 0.5 + 1e-3, dev.max()
+
+
+@pytest.mark.parametrize("name,iters", [("tgv_422_80x40", 9), ("tgv_440_48x64", 9), ("tgv_420_208x144", 7)])
+def test_generic_projection_path_bit_exact(name, iters, monkeypatch):
+    """Channel layouts without a fused projection kernel (4:2:2, 4:4:0) and the
+    per-channel kernel forced on 4:2:0."""
+    monkeypatch.setenv("ARTEFACT_GENERIC_PROJECT", "1" if name.startswith("tgv_420") else "0")
+    wl = synth.workload(name, seed=13)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case {name} iters={iters}", flush=True)
+    gpu = run_gpu(wl, iters)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, norm_mode="f64")
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
+
+
+def test_component_smaller_than_grid_bit_exact():
+    """A component whose plane is smaller than the grid without subsampling
+    (resampled with 1x1 boxes, compute_projection.rs:14) and a grid larger than
+    every component (get_fdata's clamp, coef.rs:157-160)."""
+    from artefact_amd.types import Coefficient
+    wl = synth.workload("tgv_444_48x32", seed=21)
+    y, cb, cr = wl["coefs"]
+    small = Coefficient(32, 24, 4, 3, 1, 1, cb.dct_coefs.reshape(4, 6, 64)[:3, :4].reshape(-1), cb.quant_table)
+    coefs = [y, small, cr]
+    W, H = 56, 40
+    gpu = artefact_amd.compute(3, coefs, 0.3, [0.001, 0.002, 0.003], 8, W, H, W * H)
+    ref = oracle.compute(coefs, 0.3, [0.001, 0.002, 0.003], 8, W, H, norm_mode="f64")
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
