@@ -481,7 +481,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
 
 // Gradient norm per channel from the workgroup partials (compute_step.rs:54-58): one
 // workgroup, fixed reduction order, so the result is the same on every run.
-constexpr int NF = 1024;
+constexpr int NF = 256;
 template <int C>
 __global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigned int nwg, float *norms,
                                                  double *rnorm, double *norm_sq)
@@ -491,16 +491,16 @@ __global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigne
     DD s[C];
 #pragma unroll
     for (int c = 0; c < C; c++) s[c] = DD{0.0, 0.0};
-    for (unsigned int w0 = 0; w0 < nwg; w0 += 4 * NF) {
-        double v[4][2 * C];
+    for (unsigned int w0 = 0; w0 < nwg; w0 += 8 * NF) {
+        double v[8][2 * C];
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+        for (int u = 0; u < 8; u++) {
             const unsigned int w = w0 + u * NF + tid;
 #pragma unroll
             for (int k = 0; k < 2 * C; k++) v[u][k] = w < nwg ? partials[(size_t)w * 6 + k] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; u++)
+        for (int u = 0; u < 8; u++)
 #pragma unroll
             for (int c = 0; c < C; c++) s[c] = dd_merge(s[c], DD{v[u][2 * c], v[u][2 * c + 1]});
     }
@@ -512,16 +512,16 @@ __global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigne
         if (lane == 0) red[wave][c] = v;
     }
     __syncthreads();
-    if (tid == 0) {
-        for (int c = 0; c < C; c++) {
-            DD v = red[0][c];
-            for (int w = 1; w < NF / 64; w++) v = dd_merge(v, red[w][c]);
-            const double sum = v.hi + v.lo;
-            const float norm = (float)sqrt(sum);
-            norm_sq[c] = sum;
-            norms[c] = norm;
-            rnorm[c] = norm != 0.0f ? 1.0 / (double)norm : 0.0;
-        }
+    if (tid < C) {
+        const int c = tid;
+        DD v = red[0][c];
+#pragma unroll
+        for (int w = 1; w < NF / 64; w++) v = dd_merge(v, red[w][c]);
+        const double sum = v.hi + v.lo;
+        const float norm = (float)sqrt(sum);
+        norm_sq[c] = sum;
+        norms[c] = norm;
+        rnorm[c] = norm != 0.0f ? 1.0 / (double)norm : 0.0;
     }
 }
 

@@ -30,6 +30,48 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+ITERATION_KERNELS = ("k_gradient", "k_finalize", "k_project")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tgv_420_2048x1536.json")
+
+
+def summarize_pmc(fetch_csv, write_csv, copy_bytes=256 << 20):
+    """Per-kernel HBM bytes from two rocprofv3 --pmc CSVs (FETCH_SIZE, WRITE_SIZE, KiB per
+    dispatch), each scaled by what the counter reads for the calibration copy of
+    copy_bytes (the largest __amd_rocclr_copyBuffer dispatch)."""
+    import csv
+    from collections import defaultdict
+
+    def load(path, counter):
+        per, copies = defaultdict(list), []
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            v = float(r["Counter_Value"])
+            if "copyBuffer" in r["Kernel_Name"]:
+                copies.append((int(r["Grid_Size"]), v))
+            else:
+                name = r["Kernel_Name"]
+                short = next((k for k in ITERATION_KERNELS if k in name), name.split("(")[0])
+                per[short].append(v)
+        # The calibration copies are the largest copy dispatches of the run.
+        top = sorted(v for _, v in copies)[-4:]
+        return per, (sum(top) / len(top) if top else None)
+
+    fetch, fcopy = load(fetch_csv, "FETCH_SIZE")
+    write, wcopy = load(write_csv, "WRITE_SIZE")
+    known_kib = copy_bytes / 1024.0
+    fscale = known_kib / fcopy if fcopy else 1.0
+    wscale = known_kib / wcopy if wcopy else 1.0
+    kernels = {}
+    for k in ITERATION_KERNELS:
+        if k in fetch:
+            fk = sum(fetch[k]) / len(fetch[k])
+            wk = sum(write.get(k, [0.0])) / max(len(write.get(k, [])), 1)
+            kernels[k] = {"fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
+                          "bytes": int(round((fk * fscale + wk * wscale) * 1024))}
+    return {"fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
+            "copy_fetch_kib": fcopy, "copy_write_kib": wcopy, "copy_bytes": copy_bytes,
+            "kernels": kernels, "bytes_per_iteration": sum(v["bytes"] for v in kernels.values())}
 
 
 def algorithmic_bytes(coefs, W, H):
@@ -89,7 +131,13 @@ def main() -> int:
     ap.add_argument("--workload", default="tgv_420_2048")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-sweep", action="store_true", help="also time 16/64/128 MiB copies")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print per-kernel HBM bytes from two rocprofv3 --pmc CSVs and exit")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        print(json.dumps(summarize_pmc(*args.summarize_pmc), indent=1))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU form is a design note only)", file=sys.stderr)
@@ -135,6 +183,12 @@ def main() -> int:
     finite = all(bool(np.isfinite(p).all()) for p in out)
     plan.close()
     calib = copy_calibration()
+    if calib is not None and args.copy_sweep:
+        calib["copy_GBps_by_size_MiB"] = {mb: copy_calibration(mb << 20, reps=8)["copy_GBps"] for mb in (16, 64, 128)}
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("bytes_per_iteration")
 
     res = {
         "metric": "Mpixel*iterations/s (plain TGV 4:2:0 2048x1536, 100 iterations)",
@@ -160,7 +214,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "one iteration = k_gradient + k_project",
             "algorithmic_bytes_per_iteration": B,
             "implementation_bytes_per_iteration": impl_bytes,
