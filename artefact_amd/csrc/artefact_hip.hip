@@ -250,6 +250,7 @@ struct IterArgs {
     float factor, step, alpha_tv, alpha_tgv;
     const float *x[3];
     const float *xp[3];
+    float *y[3];             // extrapolated image y = FISTA(x, x_prev), written by k_gradient
     float *g[3];
     float *xnew[3];
     ChanDev ch[3];
@@ -305,6 +306,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
     const __amdgpu_buffer_rsrc_t rx = plane_rsrc(a.x[0], pbytes * C);
     const __amdgpu_buffer_rsrc_t rxp = plane_rsrc(a.xp[0], pbytes * C);
     const __amdgpu_buffer_rsrc_t rg = plane_rsrc(a.g[0], pbytes * C);
+    const __amdgpu_buffer_rsrc_t ry = plane_rsrc(a.y[0], pbytes * C);
     const __amdgpu_buffer_rsrc_t rpr = plane_rsrc(a.ch[0].prob, a.prob_bytes);
     uint32_t poff[C];
     int hsh[C], vsh[C], crw[C], crh[C];
@@ -320,12 +322,12 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
     auto roff = [&](int t) -> uint32_t { return (xin && t >= 0 && t < H) ? (uint32_t)(t * W + x) * 4u : OOB; };
     auto chan = [&](uint32_t o, int c) -> uint32_t { return o == OOB ? OOB : o + (uint32_t)c * pbytes; };
 
-    float yprev[C], dxprev[C], dxs1[C], dys1[C], nx[C], nxp[C];
+    float yprev[C], ypp[C], dxprev[C], dxs1[C], dys1[C], nx[C], nxp[C];
     Contrib slot[3][C];
     DD acc[C];
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        yprev[c] = dxprev[c] = dxs1[c] = dys1[c] = 0.0f;
+        yprev[c] = ypp[c] = dxprev[c] = dxs1[c] = dys1[c] = 0.0f;
         acc[c] = DD{0.0, 0.0};
         const uint32_t o = chan(roff(r0 - 2), c);
         nx[c] = bload(rx, o);
@@ -440,6 +442,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                             gv += cur[c].tgy;
                         }
                         bstore(rg, chan(qo, c), gv);
+                        bstore(ry, chan(qo, c), ypp[c]);  // y of row q = t - 2
                         const float ga = acc_on ? gv : 0.0f;
                         dd_add(acc[c], (double)ga * (double)ga);
                     }
@@ -454,6 +457,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
 #pragma unroll
         for (int c = 0; c < C; c++) {
             dxprev[c] = dxt[c];
+            ypp[c] = yprev[c];
             yprev[c] = yt[c];
         }
     }
@@ -673,15 +677,14 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
     const int tr = tid >> 4, tc = (tid & 15) * 4;
     const int qy = y0 + tr, qx = x0 + tc;
     const bool live = qy < H && qx < W;
-    const float f = a.factor, step = a.step;
+    const float step = a.step;
     const size_t p = live ? (size_t)qy * W + qx : 0;
 
     // Every channel's loads first, so they are all in flight together.
-    float4 xv[C], pv[C], gv[C];
+    float4 yv[C], gv[C];
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        xv[c] = *reinterpret_cast<const float4 *>(a.x[c] + p);
-        pv[c] = *reinterpret_cast<const float4 *>(a.xp[c] + p);
+        yv[c] = *reinterpret_cast<const float4 *>(a.y[c] + p);
         gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
     }
 #pragma unroll
@@ -690,13 +693,12 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
         const float norm = a.norms[c];
         const double rn = a.rnorm[c];
         // Descent step on the extrapolated image (compute_step.rs:61-65).
-        const float xs[4] = {xv[c].x, xv[c].y, xv[c].z, xv[c].w};
-        const float ps[4] = {pv[c].x, pv[c].y, pv[c].z, pv[c].w};
+        const float ys[4] = {yv[c].x, yv[c].y, yv[c].z, yv[c].w};
         const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
         float yd[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            float y = fista(xs[k], ps[k], f);
+            float y = ys[k];
             if (norm != 0.0f) y -= step * qdiv(gs[k], rn);
             yd[k] = y;
         }
@@ -711,12 +713,11 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
 }
 
 // Compile-time geometry of a channel inside a k_project tile.
-template <int MODE> struct ModeGeo;
-template <> struct ModeGeo<PM_FULL> { static constexpr int HS = 1, VS = 1; static constexpr bool RES = false; };
-template <> struct [[maybe_unused]] ModeGeo<PM_SUB11> { static constexpr int HS = 1, VS = 1; static constexpr bool RES = true; };
-template <> struct ModeGeo<PM_SUB22> { static constexpr int HS = 2, VS = 2; static constexpr bool RES = true; };
-template <> struct ModeGeo<PM_SUB21> { static constexpr int HS = 2, VS = 1; static constexpr bool RES = true; };
-template <> struct ModeGeo<PM_SUB12> { static constexpr int HS = 1, VS = 2; static constexpr bool RES = true; };
+template <int MODE> struct ModeGeo {
+    static constexpr int HS = (MODE == PM_SUB22 || MODE == PM_SUB21) ? 2 : 1;
+    static constexpr int VS = (MODE == PM_SUB22 || MODE == PM_SUB12) ? 2 : 1;
+    static constexpr bool RES = MODE != PM_FULL;
+};
 template <int MODE> constexpr int mode_nbx() { return TW / ModeGeo<MODE>::HS / 8; }
 template <int MODE> constexpr int mode_nb() { return mode_nbx<MODE>() * (TH / ModeGeo<MODE>::VS / 8); }
 
@@ -835,14 +836,13 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
     const int tr = tid >> 4, tc = (tid & 15) * 4;
     const int qy = y0 + tr, qx = x0 + tc;
     const bool live = qy < H && qx < W;
-    const float f = a.factor, step = a.step;
+    const float step = a.step;
     const size_t p = live ? (size_t)qy * W + qx : 0;
 
-    float4 xv[C], pv[C], gv[C];
+    float4 yv[C], gv[C];
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        xv[c] = *reinterpret_cast<const float4 *>(a.x[c] + p);
-        pv[c] = *reinterpret_cast<const float4 *>(a.xp[c] + p);
+        yv[c] = *reinterpret_cast<const float4 *>(a.y[c] + p);
         gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
     }
     // Descent step on the extrapolated image (compute_step.rs:61-65).
@@ -851,12 +851,11 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
     for (int c = 0; c < C; c++) {
         const float norm = a.norms[c];
         const double rn = a.rnorm[c];
-        const float xs[4] = {xv[c].x, xv[c].y, xv[c].z, xv[c].w};
-        const float ps[4] = {pv[c].x, pv[c].y, pv[c].z, pv[c].w};
+        const float ys[4] = {yv[c].x, yv[c].y, yv[c].z, yv[c].w};
         const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            float y = fista(xs[k], ps[k], f);
+            float y = ys[k];
             if (norm != 0.0f) y -= step * qdiv(gs[k], rn);
             yd[c][k] = y;
         }
@@ -1063,6 +1062,7 @@ struct artefact_hip_plan {
     ChanDev ch[3];
     float *buf[2] = {nullptr, nullptr};  // each C*N: x and x_prev, swapped every iteration
     float *g = nullptr;                  // C*N
+    float *ybuf = nullptr;               // C*N: extrapolated image of the current iteration
     float *cos = nullptr;                // sum M_c
     float *prob = nullptr;               // sum M_c
     int16_t *coef = nullptr;             // sum M_c
@@ -1113,7 +1113,7 @@ void free_plan(artefact_hip_plan *pl)
     if (!pl) return;
     DeviceGuard guard(pl->device);
     if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
-    void *bufs[] = {pl->buf[0], pl->buf[1], pl->g,        pl->cos,     pl->prob,  pl->coef, pl->quant,
+    void *bufs[] = {pl->buf[0], pl->buf[1], pl->ybuf, pl->g,        pl->cos,     pl->prob,  pl->coef, pl->quant,
                     pl->rqq,    pl->partials, pl->norms, pl->rnorm, pl->norm_sq};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
@@ -1140,6 +1140,7 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
         const size_t off = (size_t)(c < pl->C ? c : 0) * pl->N;
         a.x[c] = pl->buf[cur] + off;
         a.xp[c] = pl->buf[1 - cur] + off;
+        a.y[c] = pl->ybuf + off;
         a.xnew[c] = pl->buf[1 - cur] + off;
         a.g[c] = pl->g + off;
         a.ch[c] = pl->ch[c < pl->C ? c : 0];
@@ -1216,16 +1217,17 @@ int launch_iterations(artefact_hip_plan *pl)
 // for R output rows, so taller strips waste less; but the grid should not spill a
 // partial extra round over the ~4 resident waves per SIMD (1024 SIMDs).  Pick the R with
 // the smallest rounds * (R + 3.5).  An override (ARTEFACT_KG_ROWS) is honoured.
-int pick_rows(uint32_t W, uint32_t H)
+int pick_rows(uint32_t W, uint32_t H, uint32_t C, bool tgv)
 {
     if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
         const int r = atoi(e);
-        if (r == 4 || r == 8 || r == 12 || r == 16) return r;
+        if (r == 4 || r == 8 || r == 12 || (r == 16 && !(C == 3 && tgv))) return r;
     }
     const uint64_t cols = (W + KG_COLS - 1) / KG_COLS, slots = 4096;
     int best = 4;
     double best_cost = 1e300;
     for (int r : {4, 8, 12, 16}) {
+        if (r == 16 && C == 3 && tgv) continue;  // would exceed 128 VGPRs and spill
         const uint64_t waves = cols * ((H + r - 1) / r);
         const double cost = (double)((waves + slots - 1) / slots) * (r + 3.5);
         if (cost < best_cost) {
@@ -1308,7 +1310,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         pl->M[c] = (size_t)chs[c].rounded_w * chs[c].rounded_h;
         mtot += pl->M[c];
     }
-    pl->rows = pick_rows(pl->W, pl->H);
+    pl->rows = pick_rows(pl->W, pl->H, pl->C, pl->tgv);
     pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * pl->rows - 1) / (WAVES * pl->rows));
     pl->grid_p = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
     const size_t nwg = (size_t)pl->grid_g.x * pl->grid_g.y;
@@ -1316,6 +1318,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     ok = ok && hipMalloc(&pl->buf[0], pl->C * pl->N * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->buf[1], pl->C * pl->N * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->g, pl->C * pl->N * sizeof(float)) == hipSuccess;
+    ok = ok && hipMalloc(&pl->ybuf, pl->C * pl->N * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->cos, mtot * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->prob, mtot * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->coef, mtot * sizeof(int16_t)) == hipSuccess;
