@@ -156,6 +156,13 @@ class Plan:
                    "artefact_hip_plan_profile")
         return {"k_gradient_ms": float(out[0]), "k_project_ms": float(out[1]), "loop_ms": float(out[2])}
 
+    def objective(self) -> tuple[float, np.ndarray]:
+        """f64 objective of the current iterate: (total, [TV, second-order, prob])."""
+        t = np.zeros(3, dtype=np.float64)
+        _lib.check(self._lib.artefact_hip_plan_objective(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                   "artefact_hip_plan_objective")
+        return float(t.sum()), t
+
     @property
     def stream(self) -> int:
         return self._lib.artefact_hip_plan_stream(self._h) or 0

@@ -50,6 +50,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -1031,6 +1032,189 @@ __global__ __launch_bounds__(NT) void k_init_cos(const ChanDev ch)
     ch.prob[i] = 0.0f;
 }
 
+// ---- Objective of an iterate (build-defined; the reference only forms the discarded
+// partial sums of compute_step_tv.rs:70-71 and compute_step_tv2.rs:88-90):
+//   F(x) = sum_p alpha n(p) + sum_p a2 n2(p)
+//        + sum_c (alpha_p,c / 2) sum_blocks sum_j ((DCT(down_c x_c)_j - c_j q_j) / q_j)^2
+// with the f32 constants of the loop and every term in f64 (n, n2 by the T1/T2 forms).
+
+// Ooura 8-point DCT pass in f64 (the oracle's double build of utils/dct.rs:100-169).
+__device__ __forceinline__ void dct8d(double v[8])
+{
+    const double c1r = 0.49039264020161522456, c1i = 0.09754516100806413392, c2r = 0.46193976625564337806,
+                 c2i = 0.19134171618254488586, c3r = 0.41573480615127261854, c3i = 0.27778511650980111237,
+                 c4r = 0.35355339059327376220, w4r = 0.70710678118654752440;
+    double x0r = v[0] + v[7], x1r = v[0] - v[7], x0i = v[2] + v[5], x1i = v[2] - v[5];
+    double x2r = v[4] + v[3], x3r = v[4] - v[3], x2i = v[6] + v[1], x3i = v[6] - v[1];
+    double xr = x0r + x2r, xi = x0i + x2i;
+    v[0] = c4r * (xr + xi);
+    v[4] = c4r * (xr - xi);
+    xr = x0r - x2r;
+    xi = x0i - x2i;
+    v[2] = c2r * xr - c2i * xi;
+    v[6] = c2r * xi + c2i * xr;
+    xr = w4r * (x1i - x3i);
+    x1i = w4r * (x1i + x3i);
+    x3i = x1i - x3r;
+    x1i += x3r;
+    x3r = x1r - xr;
+    x1r += xr;
+    v[1] = c1r * x1r - c1i * x1i;
+    v[7] = c1r * x1i + c1i * x1r;
+    v[3] = c3r * x3r - c3i * x3i;
+    v[5] = c3r * x3i + c3i * x3r;
+}
+
+struct ObjArgs {
+    uint32_t W, H, C;
+    const float *x[3];
+    ChanDev ch[3];
+    double alpha, a2, ap[3];
+    double *partials;  // [workgroups][3 terms][2]
+    unsigned int pixel_wgs;
+};
+
+__device__ __forceinline__ void block_reduce_dd(DD v, DD (*red)[3], int slot, double *out)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = dd_merge(v, dd_shfl_xor(v, m));
+    if (lane == 0) red[wave][slot] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        DD t = red[0][slot];
+        for (int w = 1; w < NT / 64; w++) t = dd_merge(t, red[w][slot]);
+        out[0] = t.hi;
+        out[1] = t.lo;
+    }
+}
+
+// TV and second-order terms: one thread per grid pixel.
+template <int C>
+__global__ __launch_bounds__(NT) void k_obj_pixels(const ObjArgs a)
+{
+    __shared__ DD red[NT / 64][3];
+    const int W = a.W, H = a.H;
+    const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+    DD tv{0.0, 0.0}, tg{0.0, 0.0};
+    if (i < (size_t)W * H) {
+        const int y = (int)(i / W), x = (int)(i % W);
+        auto px = [&](int c, int yy, int xx) -> double { return (double)a.x[c][(size_t)yy * W + xx]; };
+        auto dxy = [&](int c, int yy, int xx, double &gx, double &gy) {
+            const double v = px(c, yy, xx);
+            gx = xx < W - 1 ? px(c, yy, xx + 1) - v : 0.0;
+            gy = yy < H - 1 ? px(c, yy + 1, xx) - v : 0.0;
+        };
+        double n = 0.0, n2 = 0.0;
+#pragma unroll
+        for (int c = 0; c < C; c++) {
+            double gx, gy;
+            dxy(c, y, x, gx, gy);
+            n += gx * gx + gy * gy;
+            double gxx = 0, gyx = 0, gyy = 0, gxy = 0;
+            if (x > 0) {
+                double lx, ly;
+                dxy(c, y, x - 1, lx, ly);
+                gxx = gx - lx;
+                gyx = gy - ly;
+            }
+            if (y > 0) {
+                double ux, uy;
+                dxy(c, y - 1, x, ux, uy);
+                gyy = gy - uy;
+                gxy = gx - ux;
+            }
+            const double sm = 0.5 * (gxy + gyx);
+            n2 += gyy * gyy + 2.0 * sm * sm + gxx * gxx;
+        }
+        tv.hi = a.alpha * sqrt(n);
+        tg.hi = a.a2 * sqrt(n2);
+    }
+    block_reduce_dd(tv, red, 0, a.partials + (size_t)blockIdx.x * 6 + 0);
+    block_reduce_dd(tg, red, 1, a.partials + (size_t)blockIdx.x * 6 + 2);
+    if (threadIdx.x == 0) {
+        a.partials[(size_t)blockIdx.x * 6 + 4] = 0.0;
+        a.partials[(size_t)blockIdx.x * 6 + 5] = 0.0;
+    }
+}
+
+// Prob term: one thread per 8x8 block of every channel (box means for resampled channels).
+template <int C>
+__global__ __launch_bounds__(NT) void k_obj_blocks(const ObjArgs a)
+{
+    __shared__ DD red[NT / 64][3];
+    const size_t b = (size_t)blockIdx.x * NT + threadIdx.x;  // block b of every channel
+    DD acc{0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const ChanDev &ch = a.ch[c];
+        const size_t nb = (size_t)ch.bw * ch.bh;
+        if (b < nb) {
+            if (a.ap[c] != 0.0) {
+                const int bx = (int)(b % ch.bw), by = (int)(b / ch.bw);
+                const int hs = ch.hs, vs = ch.vs, W = a.W;
+                double blk[64];
+                for (int iy = 0; iy < 8; iy++)
+                    for (int ix = 0; ix < 8; ix++) {
+                        const int cy = by * 8 + iy, cx = bx * 8 + ix;
+                        double v;
+                        if (ch.resample) {
+                            v = 0.0;
+                            for (int sy = 0; sy < vs; sy++)
+                                for (int sx = 0; sx < hs; sx++)
+                                    v += (double)a.x[c][(size_t)(cy * vs + sy) * W + cx * hs + sx];
+                            v /= (double)(hs * vs);
+                        } else {
+                            v = (double)a.x[c][(size_t)cy * W + cx];
+                        }
+                        blk[iy * 8 + ix] = v;
+                    }
+                for (int j = 0; j < 8; j++) {
+                    double v[8];
+                    for (int k = 0; k < 8; k++) v[k] = blk[k * 8 + j];
+                    dct8d(v);
+                    for (int k = 0; k < 8; k++) blk[k * 8 + j] = v[k];
+                }
+                double s = 0.0;
+                for (int j = 0; j < 8; j++) {
+                    double v[8];
+                    for (int k = 0; k < 8; k++) v[k] = blk[j * 8 + k];
+                    dct8d(v);
+                    for (int k = 0; k < 8; k++) {
+                        const double q = (double)ch.quant[j * 8 + k];
+                        const double d = (v[k] - (double)ch.coef[b * 64 + j * 8 + k] * q) / q;
+                        s += d * d;
+                    }
+                }
+                dd_add(acc, 0.5 * a.ap[c] * s);
+            }
+        }
+    }
+    double out[2] = {0.0, 0.0};
+    block_reduce_dd(acc, red, 2, out);
+    if (threadIdx.x == 0) {
+        double *pp = a.partials + ((size_t)a.pixel_wgs + blockIdx.x) * 6;
+        pp[0] = pp[1] = pp[2] = pp[3] = 0.0;
+        pp[4] = out[0];
+        pp[5] = out[1];
+    }
+}
+
+// Fixed-order reduction of the objective partials: out[0..2] = TV, second-order, prob.
+__global__ __launch_bounds__(NT) void k_obj_final(const double *partials, unsigned int n, double *out)
+{
+    __shared__ DD red[NT / 64][3];
+    DD t[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (unsigned int w = threadIdx.x; w < n; w += NT)
+#pragma unroll
+        for (int k = 0; k < 3; k++) t[k] = dd_merge(t[k], DD{partials[(size_t)w * 6 + 2 * k], partials[(size_t)w * 6 + 2 * k + 1]});
+    double o[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) block_reduce_dd(t[k], red, k, o[k]);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) out[k] = o[k][0] + o[k][1];
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev)
@@ -1550,6 +1734,54 @@ int artefact_hip_plan_profile(artefact_hip_plan *pl, float *avg_ms)
     pl->cur = cur;
     pl->initialized = false;
     return ARTEFACT_OK;
+}
+
+int artefact_hip_plan_objective(artefact_hip_plan *pl, double *terms)
+{
+    if (!pl || !terms) return ARTEFACT_E_ARGUMENT;
+    DeviceGuard guard(pl->device);
+    ObjArgs a;
+    memset(&a, 0, sizeof(a));
+    a.W = pl->W;
+    a.H = pl->H;
+    a.C = pl->C;
+    for (uint32_t c = 0; c < 3; c++) {
+        a.x[c] = pl->buf[pl->cur] + (size_t)(c < pl->C ? c : 0) * pl->N;
+        a.ch[c] = pl->ch[c < pl->C ? c : 0];
+        a.ap[c] = c < pl->C ? (double)pl->ch[c].alpha_p : 0.0;
+    }
+    const float Cf = (float)pl->C;
+    a.alpha = (double)(1.0f / sqrtf(Cf));
+    a.a2 = (double)((pl->weight / sqrtf(2.0f)) / sqrtf(Cf));
+    const unsigned int pix_wgs = (unsigned int)((pl->N + NT - 1) / NT);
+    size_t maxb = 0;
+    for (uint32_t c = 0; c < pl->C; c++) maxb = std::max(maxb, (size_t)pl->ch[c].bw * pl->ch[c].bh);
+    const unsigned int blk_wgs = (unsigned int)((maxb + NT - 1) / NT);
+    double *part = nullptr, *out = nullptr;
+    bool ok = hipMalloc(&part, ((size_t)pix_wgs + blk_wgs) * 6 * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&out, 3 * sizeof(double)) == hipSuccess;
+    if (ok) {
+        a.partials = part;
+        a.pixel_wgs = pix_wgs;
+        if (pl->C == 1) {
+            hipLaunchKernelGGL(k_obj_pixels<1>, dim3(pix_wgs), dim3(NT), 0, pl->stream, a);
+            hipLaunchKernelGGL(k_obj_blocks<1>, dim3(blk_wgs), dim3(NT), 0, pl->stream, a);
+        } else if (pl->C == 2) {
+            hipLaunchKernelGGL(k_obj_pixels<2>, dim3(pix_wgs), dim3(NT), 0, pl->stream, a);
+            hipLaunchKernelGGL(k_obj_blocks<2>, dim3(blk_wgs), dim3(NT), 0, pl->stream, a);
+        } else {
+            hipLaunchKernelGGL(k_obj_pixels<3>, dim3(pix_wgs), dim3(NT), 0, pl->stream, a);
+            hipLaunchKernelGGL(k_obj_blocks<3>, dim3(blk_wgs), dim3(NT), 0, pl->stream, a);
+        }
+        hipLaunchKernelGGL(k_obj_final, dim3(1), dim3(NT), 0, pl->stream, (const double *)part, pix_wgs + blk_wgs,
+                           out);
+        ok = hipGetLastError() == hipSuccess;
+        ok = ok && hipMemcpyAsync(terms, out, 3 * sizeof(double), hipMemcpyDeviceToHost, pl->stream) == hipSuccess;
+        ok = ok && hipStreamSynchronize(pl->stream) == hipSuccess;
+    }
+    if (part) (void)hipFree(part);
+    if (out) (void)hipFree(out);
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
 
 void *artefact_hip_plan_stream(artefact_hip_plan *pl) { return pl ? (void *)pl->stream : nullptr; }

@@ -79,6 +79,20 @@ def algorithmic_bytes(coefs, W, H):
     return sum(20 * N + 16 * c.rounded_px_count for c in coefs)
 
 
+def subsampling(coefs):
+    if len(coefs) == 1:
+        return "luma"
+    f = (coefs[1].horizontal_samp_factor, coefs[1].vertical_samp_factor)
+    return {(1, 1): "4:4:4", (2, 2): "4:2:0", (2, 1): "4:2:2", (1, 2): "4:4:0"}.get(f, str(f))
+
+
+def describe(wl):
+    kind = "plain TV" if wl["mode"] == "tv" else "plain TGV"
+    chans = "luma only" if len(wl["coefs"]) == 1 else f"YCbCr {subsampling(wl['coefs'])}"
+    return (f"{kind}, {chans}, {wl['width']}x{wl['height']}, IJG q{wl['quality']}, {wl['iterations']} iterations, "
+            f"weight {wl['weight']}, pweight {wl['pweight'][0]}, seed {wl['seed']}")
+
+
 def copy_calibration(nbytes=256 << 20, reps=4):
     """Device-to-device copy of a known size (HIP runtime copy kernel), timed with HIP
     events: the achievable streaming bandwidth on this box, and under rocprofv3 --pmc
@@ -118,7 +132,7 @@ def cpu_baseline(wl, iters):
         "unit": "Mpixel*it/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{W}x{H} 4:2:0 TGV, {iters} iterations, loop only ({secs:.2f} s), single-threaded C restatement "
+        "sample": f"{describe(wl)} geometry, {iters} iterations, loop only ({secs:.2f} s), single-threaded C restatement "
                   f"of pipeline_scalar (oracle/oracle.c, -O2 -ffp-contract=off)",
     }
 
@@ -179,6 +193,9 @@ def main() -> int:
     N = W * H
     impl_bytes = sum(28 * N + 12 * c.rounded_px_count for c in coefs)
 
+    plan.init()
+    plan.iterate()
+    objective, terms = plan.objective()
     out = plan.download()
     finite = all(bool(np.isfinite(p).all()) for p in out)
     plan.close()
@@ -186,12 +203,12 @@ def main() -> int:
     if calib is not None and args.copy_sweep:
         calib["copy_GBps_by_size_MiB"] = {mb: copy_calibration(mb << 20, reps=8)["copy_GBps"] for mb in (16, 64, 128)}
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    if os.path.exists(TRAFFIC_FILE) and args.workload == "tgv_420_2048":
         with open(TRAFFIC_FILE) as fh:
             traffic = json.load(fh).get("bytes_per_iteration")
 
     res = {
-        "metric": "Mpixel*iterations/s (plain TGV 4:2:0 2048x1536, 100 iterations)",
+        "metric": "Mpixel*iterations/s + time to plain-100-iter objective",
         "value": round(value, 2),
         "unit": "Mpixel*it/s",
         "n_gpus": 1,
@@ -204,9 +221,12 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded generator, artefact_amd/synth.py)",
         "config": {
-            "workload": "plain TGV YCbCr 4:2:0 2048x1536, IJG q75, 100 iterations, weight 0.3, pweight 0.001",
-            "width": W, "height": H, "subsampling": "4:2:0", "iterations": iters, "quality": wl["quality"],
-            "time_to_100_iter_objective_ms": round(ms_per_step, 4),
+            "workload": describe(wl),
+            "width": W, "height": H, "subsampling": subsampling(coefs), "iterations": iters,
+            "quality": wl["quality"], "mode": wl["mode"],
+            f"time_to_{iters}_iter_objective_ms": round(ms_per_step, 4),
+            f"objective_after_{iters}_iter": objective,
+            "objective_terms_tv_tgv_prob": [float(t) for t in terms],
         },
         "roofline": {
             "bound": "hbm",

@@ -98,6 +98,12 @@ int artefact_hip_plan_first_gradient(artefact_hip_plan *plan, float *const *out_
  * iterate, it consumes the initialised state.  No reference counterpart: the
  * reference's only timing hook is its benchmark exit (lib.rs:155-157). */
 int artefact_hip_plan_profile(artefact_hip_plan *plan, float *avg_ms);
+/* Objective of the current iterate in f64: terms[0] = sum alpha*n(p) (TV form of
+ * compute_step_tv.rs:62-71), terms[1] = sum a2*n2(p) (second-order form of
+ * compute_step_tv2.rs:53-90), terms[2] = sum_c alpha_p/2 * sum ((DCT(down x)_j - c_j q_j)/q_j)^2.
+ * The reference forms only the first two as discarded partial sums (compute_step.rs:39,42);
+ * no reference counterpart for the total. */
+int artefact_hip_plan_objective(artefact_hip_plan *plan, double *terms);
 /* The plan's HIP stream (hipStream_t), for callers that record events on it. */
 void *artefact_hip_plan_stream(artefact_hip_plan *plan);
 /* Enables (1, default) or disables (0) capturing the iteration loop in a hipGraph. */
