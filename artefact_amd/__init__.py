@@ -2,9 +2,12 @@
 
 ``compute`` mirrors backend/artefact-lib/pipeline_scalar/mod.rs:17-26; ``Plan`` is
 its resident form.  ``Coefficient`` mirrors backend/artefact-lib/jpeg/mod.rs:8-27.
+``Artefact`` mirrors the public builder of backend/artefact-lib/lib.rs:49-212.
 """
 from ._lib import ArtefactError, LIB_PATH
 from .compute import Plan, compute
+from .process import Artefact, ArtefactProcessError, ForAll, ForEach, Jpeg, JpegSource, process_coefficients
 from .types import Coefficient, max_grid
 
-__all__ = ["ArtefactError", "Coefficient", "LIB_PATH", "Plan", "compute", "max_grid"]
+__all__ = ["Artefact", "ArtefactError", "ArtefactProcessError", "Coefficient", "ForAll", "ForEach", "Jpeg",
+           "JpegSource", "LIB_PATH", "Plan", "compute", "max_grid", "process_coefficients"]

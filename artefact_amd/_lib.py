@@ -21,7 +21,9 @@ ERRORS = {
     -3: "zero entry in a quantisation table",
     -4: "HIP runtime failure",
     -5: "plan used out of order",
+    1: "benchmark run: the loop ran and no image was written",
 }
+ARTEFACT_BENCHMARK = 1
 
 
 class ArtefactError(RuntimeError):
@@ -56,7 +58,21 @@ class artefact_params(ctypes.Structure):
     ]
 
 
+class artefact_process_params(ctypes.Structure):
+    _fields_ = [
+        ("nchannel", ctypes.c_uint32),
+        ("real_w", ctypes.c_uint32),
+        ("real_h", ctypes.c_uint32),
+        ("weight", ctypes.c_float * 3),
+        ("pweight", ctypes.c_float * 3),
+        ("iterations", ctypes.c_uint32 * 3),
+        ("separate_components", ctypes.c_uint32),
+        ("benchmark", ctypes.c_uint32),
+    ]
+
+
 FLOAT_PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
+U8_P = ctypes.POINTER(ctypes.c_uint8)
 
 # (name, restype, argtypes) of every exported symbol of include/artefact_hip.h.
 SIGNATURES = [
@@ -74,6 +90,9 @@ SIGNATURES = [
     ("artefact_hip_plan_stream", ctypes.c_void_p, [ctypes.c_void_p]),
     ("artefact_hip_plan_set_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     ("artefact_hip_plan_destroy", None, [ctypes.c_void_p]),
+    ("artefact_hip_plan_download_rgb", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, U8_P]),
+    ("artefact_hip_process", ctypes.c_int,
+     [ctypes.POINTER(artefact_process_params), ctypes.POINTER(artefact_channel), U8_P]),
     ("artefact_hip_strerror", ctypes.c_char_p, [ctypes.c_int]),
 ]
 

@@ -137,6 +137,14 @@ class Plan:
                    "artefact_hip_plan_download_cos")
         return out
 
+    def download_rgb(self, real_w: int, real_h: int) -> np.ndarray:
+        """Output stage on the GPU (lib.rs:159-211): uint8 (real_h, real_w, 3)."""
+        out = np.empty((real_h, real_w, 3), dtype=np.uint8)
+        _lib.check(self._lib.artefact_hip_plan_download_rgb(self._h, real_w, real_h,
+                                                            out.ctypes.data_as(_lib.U8_P)),
+                   "artefact_hip_plan_download_rgb")
+        return out
+
     def last_norms(self) -> np.ndarray:
         out = np.zeros(self.nchannel, dtype=np.float32)
         _lib.check(self._lib.artefact_hip_plan_last_norms(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),

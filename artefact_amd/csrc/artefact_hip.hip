@@ -1229,6 +1229,70 @@ struct DeviceGuard {
     }
 };
 
+// ---------------------------------------------------------------------------------------
+// Output stage (lib.rs:159-211).  Each thread converts 4 consecutive pixels of one row: the
+// planes are row-major with a stride that is a multiple of 8, so the float4 loads at a
+// column that is a multiple of 4 stay inside the row.  Products and sums are rounded one at
+// a time (-ffp-contract=off), as the reference's default mul_add! (utils/macros.rs:10-15).
+struct ColourArgs {
+    const float *y, *cb, *cr;
+    uint32_t stride, real_w, real_h, colour;
+    uint8_t *rgb;
+};
+
+// f32::clamp(0, 255) then `as u8`: NaN survives the clamp and the saturating cast maps it to 0.
+__device__ __forceinline__ uint32_t rust_u8(float v)
+{
+    v = v < 0.0f ? 0.0f : v;
+    v = v > 255.0f ? 255.0f : v;
+    return v == v ? (uint32_t)v : 0u;
+}
+
+__global__ __launch_bounds__(NT) void k_colour(ColourArgs a)
+{
+    const uint32_t j0 = (blockIdx.x * NT + threadIdx.x) * 4;
+    if (j0 >= a.real_w) return;
+    for (uint32_t i = blockIdx.y; i < a.real_h; i += gridDim.y) {
+        const size_t row = (size_t)i * a.stride + j0;
+        const float4 yv = *reinterpret_cast<const float4 *>(a.y + row);
+        float4 cbv = make_float4(0.f, 0.f, 0.f, 0.f), crv = cbv;
+        if (a.colour) {
+            cbv = *reinterpret_cast<const float4 *>(a.cb + row);
+            crv = *reinterpret_cast<const float4 *>(a.cr + row);
+        }
+        const float ys[4] = {yv.x, yv.y, yv.z, yv.w};
+        const float cbs[4] = {cbv.x, cbv.y, cbv.z, cbv.w};
+        const float crs[4] = {crv.x, crv.y, crv.z, crv.w};
+        uint32_t bytes[12];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float y = ys[k] + 128.0f;
+            if (a.colour) {
+                const float r = 1.402f * crs[k] + y;
+                const float gi = 0.34414f * -cbs[k] + y;
+                const float g = 0.71414f * -crs[k] + gi;
+                const float b = 1.772f * cbs[k] + y;
+                bytes[3 * k + 0] = rust_u8(r);
+                bytes[3 * k + 1] = rust_u8(g);
+                bytes[3 * k + 2] = rust_u8(b);
+            } else {
+                const uint32_t v = rust_u8(y);
+                bytes[3 * k + 0] = bytes[3 * k + 1] = bytes[3 * k + 2] = v;
+            }
+        }
+        const size_t o = ((size_t)i * a.real_w + j0) * 3;
+        if (j0 + 4 <= a.real_w && (o & 3) == 0) {
+            uint32_t *dst = reinterpret_cast<uint32_t *>(a.rgb + o);
+#pragma unroll
+            for (int w = 0; w < 3; w++)
+                dst[w] = bytes[4 * w] | bytes[4 * w + 1] << 8 | bytes[4 * w + 2] << 16 | bytes[4 * w + 3] << 24;
+        } else {
+            const uint32_t n = min(4u, a.real_w - j0) * 3;
+            for (uint32_t k = 0; k < n; k++) a.rgb[o + k] = (uint8_t)bytes[k];
+        }
+    }
+}
+
 }  // namespace
 
 struct artefact_hip_plan {
@@ -1420,6 +1484,21 @@ int pick_rows(uint32_t W, uint32_t H, uint32_t C, bool tgv)
         }
     }
     return best;
+}
+
+// Output stage on `stream` from device planes (y, cb, cr; cb/cr ignored unless colour) into
+// host memory, staging the bytes in `staging` (at least real_w*real_h*3 bytes of device
+// memory).  Synchronises the stream.
+int run_colour(hipStream_t stream, const float *y, const float *cb, const float *cr, uint32_t stride,
+               uint32_t real_w, uint32_t real_h, bool colour, uint8_t *staging, uint8_t *out)
+{
+    ColourArgs a{y, cb, cr, stride, real_w, real_h, colour ? 1u : 0u, staging};
+    const dim3 grid((real_w + 4 * NT - 1) / (4 * NT), std::min(real_h, 65535u));
+    hipLaunchKernelGGL(k_colour, grid, dim3(NT), 0, stream, a);
+    bool ok = hipGetLastError() == hipSuccess;
+    ok = ok && hipMemcpyAsync(out, staging, (size_t)real_w * real_h * 3, hipMemcpyDeviceToHost, stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(stream) == hipSuccess;
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
 
 }  // namespace
@@ -1801,6 +1880,69 @@ void artefact_hip_plan_destroy(artefact_hip_plan *pl)
         if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     }
     free_plan(pl);
+}
+
+int artefact_hip_plan_download_rgb(artefact_hip_plan *pl, uint32_t real_w, uint32_t real_h, uint8_t *out)
+{
+    if (!pl || !out) return ARTEFACT_E_ARGUMENT;
+    if (!real_w || !real_h || real_w > pl->W || real_h > pl->H) return ARTEFACT_E_GEOMETRY;
+    DeviceGuard guard(pl->device);
+    const float *x = pl->buf[pl->cur];
+    // The gradient buffer (C*N floats >= 4N bytes) stages the 3*real_w*real_h bytes.
+    return run_colour(pl->stream, x, x + (pl->C > 1 ? pl->N : 0), x + (pl->C > 2 ? 2 * pl->N : 0), pl->W, real_w,
+                      real_h, pl->C == 3, reinterpret_cast<uint8_t *>(pl->g), out);
+}
+
+int artefact_hip_process(const artefact_process_params *p, const artefact_channel *ch, uint8_t *out)
+{
+    if (!p || !ch) return ARTEFACT_E_ARGUMENT;
+    if (p->nchannel < 1 || p->nchannel > 3) return ARTEFACT_E_ARGUMENT;
+    if (!p->benchmark && !out) return ARTEFACT_E_ARGUMENT;
+    uint32_t W = 0, H = 0;
+    for (uint32_t c = 0; c < p->nchannel; c++) {
+        W = std::max(W, ch[c].rounded_w);
+        H = std::max(H, ch[c].rounded_h);
+    }
+    if (!p->real_w || !p->real_h || p->real_w > W || p->real_h > H) return ARTEFACT_E_GEOMETRY;
+    const bool joint = p->nchannel == 3 && !p->separate_components;
+    const uint32_t nplan = joint ? 1 : p->nchannel;
+    artefact_hip_plan *plans[3] = {nullptr, nullptr, nullptr};
+    int rc = ARTEFACT_OK;
+    for (uint32_t k = 0; k < nplan && !rc; k++) {
+        artefact_params q{};
+        q.nchannel = joint ? 3 : 1;
+        q.max_w = W;
+        q.max_h = H;
+        q.weight = p->weight[k];
+        for (int c = 0; c < 3; c++) q.pweight[c] = p->pweight[c];
+        q.iterations = p->iterations[k];
+        q.mode = ARTEFACT_MODE_TGV;
+        q.init_override = nullptr;
+        rc = artefact_hip_plan_create(&q, ch + k, &plans[k]);
+        if (!rc) plans[k]->use_graph = false;
+    }
+    // Every loop is queued before any is awaited, so separate components overlap on the GPU.
+    for (uint32_t k = 0; k < nplan && !rc; k++) rc = artefact_hip_plan_init(plans[k]);
+    for (uint32_t k = 0; k < nplan && !rc; k++) rc = artefact_hip_plan_iterate(plans[k]);
+    for (uint32_t k = 1; k < nplan && !rc; k++) {
+        DeviceGuard guard(plans[k]->device);
+        if (hipStreamSynchronize(plans[k]->stream) != hipSuccess) rc = ARTEFACT_E_HIP;
+    }
+    if (!rc && p->benchmark) {
+        DeviceGuard guard(plans[0]->device);
+        rc = hipStreamSynchronize(plans[0]->stream) == hipSuccess ? ARTEFACT_BENCHMARK : ARTEFACT_E_HIP;
+    } else if (!rc) {
+        artefact_hip_plan *p0 = plans[0];
+        DeviceGuard guard(p0->device);
+        const float *planes[3];
+        for (uint32_t c = 0; c < p->nchannel; c++)
+            planes[c] = joint ? p0->buf[p0->cur] + (size_t)c * p0->N : plans[c]->buf[plans[c]->cur];
+        for (uint32_t c = p->nchannel; c < 3; c++) planes[c] = planes[0];
+        rc = run_colour(p0->stream, planes[0], planes[1], planes[2], W, p->real_w, p->real_h, p->nchannel == 3,
+                        reinterpret_cast<uint8_t *>(p0->g), out);
+    }
+    for (uint32_t k = 0; k < nplan; k++) artefact_hip_plan_destroy(plans[k]);
+    return rc;
 }
 
 int artefact_hip_compute(const artefact_params *p, const artefact_channel *ch, float *const *out)

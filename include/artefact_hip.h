@@ -32,6 +32,8 @@ extern "C" {
 #define ARTEFACT_E_QUANT        -3  /* a quantisation table entry is 0 (the reference divides by q^2, compute_step_prob.rs:29) */
 #define ARTEFACT_E_HIP          -4  /* HIP runtime failure (allocation, launch, copy) */
 #define ARTEFACT_E_STATE        -5  /* plan used out of order (iterate before init) */
+#define ARTEFACT_BENCHMARK       1  /* artefact_hip_process with benchmark set: the loop ran, no image was
+                                       written (the reference returns Err("BENCHMARK"), lib.rs:155-157) */
 
 #define ARTEFACT_MODE_TGV 0  /* TV + second-order term with strength weight (compute_step.rs:42-48) */
 #define ARTEFACT_MODE_TV  1  /* TV only: the same loop with weight forced to 0 */
@@ -109,6 +111,40 @@ void *artefact_hip_plan_stream(artefact_hip_plan *plan);
 /* Enables (1, default) or disables (0) capturing the iteration loop in a hipGraph. */
 int artefact_hip_plan_set_graph(artefact_hip_plan *plan, int enable);
 void artefact_hip_plan_destroy(artefact_hip_plan *plan);
+
+/* Output stage of Artefact::process (lib.rs:159-211), run on the GPU from the plan's
+ * current planes: luma + 128, then for three channels R = 1.402*cr + y,
+ * G = 0.71414*(-cr) + (0.34414*(-cb) + y), B = 1.772*cb + y with every product and sum
+ * rounded on its own (utils/macros.rs:10-15), clamp to [0, 255] and truncation to u8 (NaN
+ * gives 0, as Rust's `as u8`); one or two channels give grey R = G = B from channel 0.
+ * Writes real_w*real_h*3 bytes, row-major, interleaved RGB, for the top-left
+ * real_w x real_h of the grid (lib.rs:168-172).  Uses the plan's gradient buffer as staging,
+ * so it must not run while an iterate on the same plan is in flight (it synchronises the
+ * plan's stream before returning). */
+int artefact_hip_plan_download_rgb(artefact_hip_plan *plan, uint32_t real_w, uint32_t real_h, uint8_t *out_rgb);
+
+/* The arguments of Artefact::process once the JPEG is decoded (lib.rs:49-57, 99-157):
+ * ValueCollection::to_slice already applied (lib.rs:41-46). */
+typedef struct {
+    uint32_t nchannel;              /* Jpeg::nchannel, 1..3 */
+    uint32_t real_w, real_h;        /* Jpeg::real_px_w / real_px_h */
+    float weight[3];
+    float pweight[3];
+    uint32_t iterations[3];
+    uint32_t separate_components;   /* nonzero: one single-channel loop per component */
+    uint32_t benchmark;             /* nonzero: run the loops, write nothing, return ARTEFACT_BENCHMARK */
+} artefact_process_params;
+
+/* Artefact::process from decoded coefficients to the RGB image (lib.rs:105-211).  The max
+ * grid is the largest component (lib.rs:108-116).  Three channels, not separate: one joint
+ * loop with weight[0], pweight, iterations[0] (lib.rs:121-132).  Otherwise one loop per
+ * component c with nchannel = 1, weight[c], iterations[c] and the whole pweight array,
+ * so every component uses pweight[0] (lib.rs:134-149, compute_step.rs:26,30); the
+ * per-component loops run concurrently, each on its own stream.  Then the output stage of
+ * artefact_hip_plan_download_rgb into out_rgb (real_w*real_h*3 bytes; may be NULL when
+ * benchmark is set). */
+int artefact_hip_process(const artefact_process_params *params, const artefact_channel *channels,
+                         uint8_t *out_rgb);
 
 const char *artefact_hip_strerror(int code);
 

@@ -60,6 +60,7 @@ _SIGS = {
     "oracle_fista_scalars": (None, [_u32, _u32, _F, _F, _F]),
     "oracle_objective": (ctypes.c_double, [_u32, _CH, ctypes.c_float, _F, _u32, _u32, _FPP, _D]),
     "oracle_objective_f64planes": (ctypes.c_double, [_u32, _CH, ctypes.c_float, _F, _u32, _u32, _DPP, _D]),
+    "oracle_output_rgb": (None, [_u32, _FPP, _u32, _u32, _u32, ctypes.POINTER(ctypes.c_uint8)]),
 }
 
 _lib = None
@@ -276,3 +277,36 @@ def objective(coefs, weight, pweight, max_w, max_h, planes):
         ps = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in planes]
         tot = lib.oracle_objective(C, chans, float(weight), _fptr(pw), max_w, max_h, _fpp(ps), _dptr(terms))
     return tot, terms
+
+
+def output_rgb(planes, max_w, real_w, real_h):
+    """Output stage of backend/artefact-lib/lib.rs:159-211 on loop planes (luma level-shifted).
+
+    Returns uint8 (real_h, real_w, 3); grey (R = G = B) unless there are three planes.
+    """
+    lib = load()
+    ps = [np.ascontiguousarray(a, dtype=np.float32).reshape(-1) for a in planes]
+    out = np.empty((real_h, real_w, 3), dtype=np.uint8)
+    lib.oracle_output_rgb(len(ps), _fpp(ps), max_w, real_w, real_h,
+                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    return out
+
+
+def process(coefs, real_w, real_h, weight, pweight, iterations, separate_components=False, *,
+            norm_mode="reference", init_mode="corrected"):
+    """Artefact::process after decoding (backend/artefact-lib/lib.rs:105-211).
+
+    ``weight``, ``pweight`` and ``iterations`` are 3-element lists (ValueCollection::to_slice,
+    lib.rs:41-46).  Three components and not ``separate_components``: one joint loop with
+    weight[0] and iterations[0] (:121-132).  Otherwise one single-channel loop per component with
+    weight[c] and iterations[c]; each receives the whole pweight array and so uses pweight[0]
+    (:134-149 with compute_step.rs:19-30).  Returns uint8 (real_h, real_w, 3).
+    """
+    W = max(c.rounded_px_w for c in coefs)
+    H = max(c.rounded_px_h for c in coefs)
+    if len(coefs) == 3 and not separate_components:
+        planes = compute(coefs, weight[0], pweight, iterations[0], W, H, norm_mode=norm_mode, init_mode=init_mode)
+    else:
+        planes = [compute([c], weight[i], pweight, iterations[i], W, H, norm_mode=norm_mode,
+                          init_mode=init_mode)[0] for i, c in enumerate(coefs)]
+    return output_rgb(planes, W, real_w, real_h)

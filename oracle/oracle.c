@@ -24,6 +24,8 @@
  *   compute                 special cases: iterations=0 returns the init, weight=0 is the
  *                           TV-only loop, uniform-DC image never moves
  *   objective               closed-form values on constant / ramp planes
+ *   output_rgb              inverse of the BT.601 full-range forward transform (round trip
+ *                           within the truncation), clamp / NaN / truncation special cases
  */
 #include "oracle.h"
 
@@ -263,4 +265,42 @@ double oracle_objective(uint32_t C, const oracle_channel *ch, float weight, cons
     double r = objective_impl(C, ch, weight, pweight, W, H, (const double *const *)x, terms);
     for (uint32_t c = 0; c < C; c++) free(x[c]);
     return r;
+}
+
+/* Output stage, backend/artefact-lib/lib.rs:159-211: luma gets +128 in f32 (:160-162); for three
+ * channels each pixel becomes R = 1.402*cr + y, G = 0.71414*(-cr) + (0.34414*(-cb) + y),
+ * B = 1.772*cb + y, each product and sum rounded to f32 on its own (utils/macros.rs:10-15, the
+ * default without the `native` feature), clamped to [0, 255] (f32::clamp keeps NaN) and cast with
+ * Rust's saturating `as u8` (truncation, NaN -> 0).  Other channel counts give grey R = G = B from
+ * channel 0 (:188-210).  Rows i < real_h, columns j < real_w of the max grid (stride max_w);
+ * rgb is real_w*real_h*3 bytes, row-major, interleaved. */
+static uint8_t rust_u8(float v)
+{
+    if (v < 0.0f) v = 0.0f;
+    if (v > 255.0f) v = 255.0f;
+    if (v != v) return 0;
+    return (uint8_t)v;
+}
+
+void oracle_output_rgb(uint32_t nchannel, const float *const *planes, uint32_t max_w, uint32_t real_w,
+                       uint32_t real_h, uint8_t *rgb)
+{
+    for (uint32_t i = 0; i < real_h; i++)
+        for (uint32_t j = 0; j < real_w; j++) {
+            const size_t idx = (size_t)i * max_w + j;
+            uint8_t *o = rgb + ((size_t)i * real_w + j) * 3;
+            const float y = planes[0][idx] + 128.0f;
+            if (nchannel == 3) {
+                const float cb = planes[1][idx], cr = planes[2][idx];
+                const float r = 1.402f * cr + y;
+                const float gi = 0.34414f * -cb + y;
+                const float g = 0.71414f * -cr + gi;
+                const float b = 1.772f * cb + y;
+                o[0] = rust_u8(r);
+                o[1] = rust_u8(g);
+                o[2] = rust_u8(b);
+            } else {
+                o[0] = o[1] = o[2] = rust_u8(y);
+            }
+        }
 }

@@ -47,6 +47,10 @@ void oracle_unboxing_f32(const float *in, float *out, uint32_t rw, uint32_t bw, 
 float oracle_midpoint_f32(float a, float b);
 void oracle_fista_scalars(uint32_t iterations, uint32_t pixel_count, float *next_terms, float *factors,
                           float *step_size);
+/* Output stage of backend/artefact-lib/lib.rs:159-211 (+128 on luma, YCbCr -> RGB, clamp,
+ * truncating u8); nchannel != 3 gives grey.  rgb: real_w*real_h*3 bytes. */
+void oracle_output_rgb(uint32_t nchannel, const float *const *planes, uint32_t max_w, uint32_t real_w,
+                       uint32_t real_h, uint8_t *rgb);
 double oracle_objective(uint32_t nchannel, const oracle_channel *ch, float weight, const float *pweight,
                         uint32_t max_w, uint32_t max_h, const float *const *planes, double *terms);
 double oracle_objective_f64planes(uint32_t nchannel, const oracle_channel *ch, float weight,

@@ -1,0 +1,73 @@
+"""GPU output stage and Artefact::process against the oracle, through the C ABI.
+
+The loop is bit-exact against the oracle in norm_mode="f64", and the output stage
+is integer-valued, so every byte must match.
+"""
+import numpy as np
+import pytest
+
+import artefact_amd
+import oracle
+from artefact_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def jpeg_of(wl):
+    return artefact_amd.Jpeg(len(wl["coefs"]), wl["width"], wl["height"], wl["coefs"])
+
+
+@pytest.mark.parametrize("name,iters", [("tgv_420_93x61", 6), ("tgv_444_64x40", 3), ("tv_luma_37x21", 4),
+                                        ("tgv_420_128x64", 2)])
+def test_download_rgb_matches_oracle(name, iters):
+    wl = synth.workload(name, seed=21)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case {name} iters={iters}", flush=True)
+    with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"]) as plan:
+        plan.init()
+        plan.iterate()
+        planes = plan.download()
+        rgb = plan.download_rgb(wl["width"], wl["height"])
+    assert np.array_equal(rgb, oracle.output_rgb(planes, W, wl["width"], wl["height"]))
+
+
+@pytest.mark.parametrize("real_w", [1, 2, 3, 4, 5, 61, 64])
+def test_download_rgb_special_values_and_ragged_rows(real_w):
+    W, H = 64, 8
+    rng = np.random.default_rng(real_w)
+    y = rng.uniform(-300, 300, (H, W)).astype(np.float32)
+    cb = rng.uniform(-300, 300, (H, W)).astype(np.float32)
+    cr = rng.uniform(-300, 300, (H, W)).astype(np.float32)
+    y[0, :8] = [np.nan, -0.01, 126.99, 127.0, -128.0, np.inf, -np.inf, 0.0]
+    cb[1, :3] = [np.nan, np.inf, -np.inf]
+    wl = synth.workload("tgv_444_64x8", seed=1)
+    print(f"case special real_w={real_w}", flush=True)
+    with artefact_amd.Plan(wl["coefs"], 0.3, wl["pweight"], 0, W, H, init_override=[y, cb, cr]) as plan:
+        plan.init()
+        plan.iterate()
+        rgb = plan.download_rgb(real_w, 7)
+    assert np.array_equal(rgb, oracle.output_rgb([y, cb, cr], W, real_w, 7))
+
+
+@pytest.mark.parametrize("name,sep", [("tgv_420_93x61", False), ("tgv_420_93x61", True), ("tgv_444_40x24", True),
+                                      ("tv_luma_37x21", False)])
+def test_process_matches_oracle(name, sep):
+    wl = synth.workload(name, seed=8)
+    weight, pweight, iters = [0.3, 0.2, 0.1], [0.001, 0.01, 0.002], [5, 3, 2]
+    print(f"case process {name} separate={sep}", flush=True)
+    img = artefact_amd.process_coefficients(jpeg_of(wl), weight, pweight, iters, sep)
+    ref = oracle.process(wl["coefs"], wl["width"], wl["height"], weight, pweight, iters, sep, norm_mode="f64")
+    assert np.array_equal(img, ref)
+
+
+def test_artefact_builder_defaults_and_benchmark():
+    wl = synth.workload("tgv_420_48x32", seed=5)
+    img = artefact_amd.Artefact().source(jpeg_of(wl)).process()
+    ref = oracle.process(wl["coefs"], 48, 32, [0.3] * 3, [0.001] * 3, [50] * 3, norm_mode="f64")
+    assert np.array_equal(img, ref)
+    a = (artefact_amd.Artefact().source(jpeg_of(wl)).weight(artefact_amd.ForEach((0.3, 0.3, 0.3)))
+         .iterations(artefact_amd.ForAll(2)).benchmark(True))
+    with pytest.raises(artefact_amd.ArtefactProcessError, match="BENCHMARK"):
+        a.process()
+    with pytest.raises(artefact_amd.ArtefactProcessError, match="Source is not set"):
+        artefact_amd.Artefact().process()
