@@ -52,6 +52,7 @@
 #include <cstring>
 #include <algorithm>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/artefact_hip.h"
@@ -165,6 +166,19 @@ __device__ __forceinline__ float midpoint(float a, float b)
     return (float)(((double)a + (double)b) * 0.5);
 }
 
+// midpoint() with an f32 fast path.  When the f32 sum is finite and its half is a normal
+// number or zero, fl32(a + b) / 2 is exact and equals the widened form: either a + b is
+// exact in f64, or |b| < 2^-29 |a| and both forms round to a.  Otherwise (a subnormal or
+// infinite half, NaN) the whole wave takes the widened form.
+__device__ __forceinline__ float midpoint_fast(float a, float b)
+{
+    float m = (a + b) * 0.5f;
+    // v_cmp_class mask: NaN (bits 0-1), +-inf (2, 9), +-subnormal (4, 7).
+    const bool slow = __builtin_amdgcn_class(m, 0x297);
+    if (__builtin_amdgcn_ballot_w64(slow)) m = slow ? midpoint(a, b) : m;
+    return m;
+}
+
 // Rust's f32::clamp(lo, hi): lower bound first, NaN passes through.
 __device__ __forceinline__ float clamp_ref(float v, float lo, float hi)
 {
@@ -259,6 +273,7 @@ struct IterArgs {
     float *norms;            // [3]
     double *rnorm;           // [3] 1/norm, correctly rounded f64
     double *norm_sq;         // [3]
+    uint32_t kg_rows;        // k_gradient output rows per wave
 };
 
 // Contributions of one source pixel: TV self/right/down, second-order self/x/y/diagonal.
@@ -271,38 +286,34 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, uint
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
 }
-__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t off)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
-}
-__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t off, float v)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
-}
-constexpr uint32_t OOB = 0xFFFFFFF0u;  // a byte offset no plane reaches (planes hold < 2^31 bytes)
 
-// k_gradient: C channels, TGV on/off, R output rows per wave.  Every wave walks rows
-// r0-2 .. r0+R+1 of its 64 columns (2 halo columns each side); the row loop is fully
-// unrolled so the three-row window of contributions lives in fixed registers.
-template <int C, bool TGV, int R>
+template <int N>
+using IC = std::integral_constant<int, N>;
+
+// k_gradient: C channels, TGV on/off.  Every wave owns a strip of a.kg_rows output rows of
+// its 64 columns (2 halo columns each side) and walks rows r0-2 .. r0+kg_rows+1 once: a
+// four-step prologue, then a loop unrolled by three so that the three-row ring of
+// contributions sits in fixed registers.  The host sizes the strips so that the whole grid
+// is resident at once (one round of waves), which keeps the 2-row halo cost per strip low.
+template <int C, bool TGV>
 __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
 {
-    __shared__ DD red[WAVES][3];
+    __shared__ DD lred[WAVES][3][64];
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int W = a.W, H = a.H;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int W = a.W, H = a.H, RW = (int)a.kg_rows;
     unsigned int tx, ty;
     xcd_tile(tx, ty);
     const int x = (int)tx * KG_COLS - 2 + lane;
-    const int r0 = ((int)ty * WAVES + wave) * R;
+    const int r0 = ((int)ty * WAVES + wave) * RW;
     const bool xin = x >= 0 && x < W;
     const bool out_lane = lane >= 2 && lane < 62 && xin;
     const bool xl = x > 0;
-    const bool xr = x < W - 1;
     const float f = a.factor, atv = a.alpha_tv, atg = a.alpha_tgv;
     const uint32_t pbytes = (uint32_t)((size_t)W * H * 4);
 
-    // One buffer resource per arena (the C planes of x, x_prev and g are contiguous, the
+    // One buffer resource per arena (the C planes of x, x_prev, g and y are contiguous, the
     // prob planes too); channel c sits c * pbytes (prob: poff[c]) bytes in.
     const __amdgpu_buffer_rsrc_t rx = plane_rsrc(a.x[0], pbytes * C);
     const __amdgpu_buffer_rsrc_t rxp = plane_rsrc(a.xp[0], pbytes * C);
@@ -319,48 +330,87 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
         crw[c] = a.ch[c].rw;
         crh[c] = a.ch[c].rh;
     }
-    // Byte offset of (row t, this lane's column) in channel 0; OOB outside the image.
-    auto roff = [&](int t) -> uint32_t { return (xin && t >= 0 && t < H) ? (uint32_t)(t * W + x) * 4u : OOB; };
-    auto chan = [&](uint32_t o, int c) -> uint32_t { return o == OOB ? OOB : o + (uint32_t)c * pbytes; };
 
-    float yprev[C], ypp[C], dxprev[C], dxs1[C], dys1[C], nx[C], nxp[C];
+    // Row state in rings of three, indexed by k mod 3 so that no register moves between
+    // steps: y of rows t, t-1, t-2; x-differences of rows t, t-1, t-2; y-differences of
+    // source rows s = t-1 and s-1.
+    float yring[3][C], dxring[3][C], dyring[3][C], pring[3][C], nx[C], nxp[C];
     Contrib slot[3][C];
     DD acc[C];
+
+    // Addressing: the lane part of every offset is constant and the row and channel part is
+    // wave-uniform (the scalar offset).  Loads clamp the column to [0, W-1] and the row to
+    // [0, H-1]: the replicated edge makes the reference's zero differences at x = W-1 and
+    // y = H-1 come out of the subtractions themselves (y - y = +0), and every value a lane
+    // outside the image holds is finite and feeds only contributions that are zeroed.  Stores
+    // and prob loads use the arena size as the lane offset outside the image, which the
+    // buffer range check (on the lane offset) turns into a dropped store or a zero load.
+    const uint32_t arena = pbytes * C;
+    const uint32_t vin = (uint32_t)min(max(x, 0), W - 1) * 4u;
+    const uint32_t vout = out_lane ? (uint32_t)x * 4u : arena;
+    uint32_t vpr[C];
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        yprev[c] = ypp[c] = dxprev[c] = dxs1[c] = dys1[c] = 0.0f;
-        acc[c] = DD{0.0, 0.0};
-        const uint32_t o = chan(roff(r0 - 2), c);
-        nx[c] = bload(rx, o);
-        nxp[c] = bload(rxp, o);
+        const int cx = x >> hsh[c];
+        vpr[c] = (out_lane && cx < crw[c]) ? poff[c] + (uint32_t)cx * 4u : a.prob_bytes;
     }
-
-#pragma unroll
-    for (int k = 0; k < R + 4; k++) {
-        const int t = r0 - 2 + k;
-        float yt[C], dxt[C];
-#pragma unroll
-        for (int c = 0; c < C; c++) yt[c] = fista(nx[c], nxp[c], f);
-        if (k + 1 < R + 4) {
-            const uint32_t o = roff(t + 1);
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                nx[c] = bload(rx, chan(o, c));
-                nxp[c] = bload(rxp, chan(o, c));
-            }
-        }
+    auto rowload = [&](int t) {
+        const uint32_t tr = (uint32_t)min(max(t, 0), H - 1);
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            const float yr = from_right(yt[c]);
-            dxt[c] = xr ? yr - yt[c] : 0.0f;
+            const int so = (int)(tr * (uint32_t)W * 4u + (uint32_t)c * pbytes);
+            nx[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, (int)vin, so, 0));
+            nxp[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rxp, (int)vin, so, 0));
         }
-        if (k >= 1) {
-            const int s = t - 1;  // source row whose dy is now known
-            float dys[C];
+    };
+    // Prob term of output row q, one step ahead of its use.  A row outside the component moves
+    // the lane offset out of range (a zero load) instead of branching, so the wait for it can
+    // sit after the next row's loads.
+    auto probload = [&](int q, float *dst) {
 #pragma unroll
-            for (int c = 0; c < C; c++) dys[c] = s < H - 1 ? yt[c] - yprev[c] : 0.0f;
-            if (k >= 2) {
-                const bool valid = xin && s >= 0 && s < H;
+        for (int c = 0; c < C; c++) {
+            const int cy = q >> vsh[c];
+            const bool cin = cy < crh[c];
+            dst[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                   rpr, (int)(cin ? vpr[c] : a.prob_bytes), cin ? cy * crw[c] * 4 : 0, 0));
+        }
+    };
+
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) yring[j][c] = dxring[j][c] = dyring[j][c] = pring[j][c] = 0.0f;
+        acc[c] = DD{0.0, 0.0};
+    }
+    rowload(r0 - 2);
+    const int kend = RW + 4;  // steps k = 0 .. RW + 3; step k reads row r0 - 2 + k
+
+    // Step k: extrapolate row t = r0 - 2 + k, form its x-differences and the y-differences of
+    // row t - 1, evaluate the contributions of source row t - 1 (k >= 2) and gather output row
+    // t - 3 (k >= 4).  KM = k mod 3 picks the ring slots; PH = min(k, 4) the prologue phase.
+    // From k = 4 on the source row is >= 1, so the top-border tests drop out.
+    auto step = [&](const int k, auto KM, auto PH) {
+        constexpr int km = decltype(KM)::value, ph = decltype(PH)::value;
+        constexpr int k1 = (km + 2) % 3, k2 = (km + 1) % 3;  // rings: one and two rows back
+        const int t = r0 - 2 + k;
+        float *yt = yring[km], *dxt = dxring[km];
+#pragma unroll
+        for (int c = 0; c < C; c++) yt[c] = fista(nx[c], nxp[c], f);
+        if (k + 1 < kend) {
+            rowload(t + 1);
+            if constexpr (ph >= 3) probload(t - 1, pring[k2]);  // output row t - 1 of step k + 1
+        }
+#pragma unroll
+        for (int c = 0; c < C; c++) dxt[c] = from_right(yt[c]) - yt[c];
+        if constexpr (ph >= 1) {
+            const int s = t - 1;  // source row whose dy is now known
+            const float *yprev = yring[k1], *dxprev = dxring[k1], *dxs1 = dxring[k2], *dys1 = dyring[k1];
+            float *dys = dyring[km];
+#pragma unroll
+            for (int c = 0; c < C; c++) dys[c] = yt[c] - yprev[c];
+            if constexpr (ph >= 2) {
+                const bool top = ph < 4 && s <= 0;  // s == 0 has no row above; s < 0 is outside
+                const bool valid = xin && s < H && (ph >= 4 || s >= 0);
                 // Joint norms at source (s, x): compute_step_tv.rs:62-68, compute_step_tv2.rs:53-86.
                 float nacc = 0.0f, n2acc = 0.0f;
                 float gxx[C], gyy[C], sym[C];
@@ -374,9 +424,9 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                         const float lgx = from_left(gx), lgy = from_left(gy);
                         const float xx = xl ? gx - lgx : 0.0f;
                         const float yx = xl ? gy - lgy : 0.0f;
-                        const float yy = s > 0 ? gy - dys1[c] : 0.0f;
-                        const float xy = s > 0 ? gx - dxs1[c] : 0.0f;
-                        const float sm = midpoint(xy, yx);
+                        const float yy = top ? 0.0f : gy - dys1[c];
+                        const float xy = top ? 0.0f : gx - dxs1[c];
+                        const float sm = midpoint_fast(xy, yx);
                         n2acc += yy * yy + (2.0f * (sm * sm) + xx * xx);
                         gxx[c] = xx;
                         gyy[c] = yy;
@@ -393,7 +443,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                     const float n2 = sqrtf(n2acc);
                     rn2 = (valid && n2 != 0.0f) ? recip64(n2) : 0.0;
                 }
-                Contrib *cur = slot[k % 3];
+                Contrib *cur = slot[km];
 #pragma unroll
                 for (int c = 0; c < C; c++) {
                     const float gx = dxprev[c], gy = dys[c];
@@ -410,19 +460,21 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                     }
                     cur[c] = kc;
                 }
-                if (k >= 4) {
+                if constexpr (ph >= 4) {
                     // Gather into q = s - 1 in the reference's accumulation order.
                     const int q = s - 1;
-                    const Contrib *m1 = slot[(k + 2) % 3];  // row q = s - 1
-                    const Contrib *m2 = slot[(k + 1) % 3];  // row q - 1 = s - 2
+                    const Contrib *m1 = slot[k1];  // row q = s - 1
+                    const Contrib *m2 = slot[k2];  // row q - 1 = s - 2
+                    const float *yq = yring[k2];   // y of row q = t - 2
                     const bool qin = q < H;
-                    const uint32_t qo = out_lane ? roff(q) : OOB;
                     const bool acc_on = out_lane && qin;
+                    // Branch-free: a row outside the image moves the lane offset out of range
+                    // (a dropped store) instead, so no wait is forced at a join.
+                    const uint32_t vst = qin ? vout : arena;
+                    const uint32_t qoff = qin ? (uint32_t)q * (uint32_t)W * 4u : 0u;
 #pragma unroll
                     for (int c = 0; c < C; c++) {
-                        const int cy = q >> vsh[c], cx = x >> hsh[c];
-                        const bool pin = out_lane && cx < crw[c] && cy < crh[c];
-                        const float pv = bload(rpr, pin ? poff[c] + (uint32_t)(cy * crw[c] + cx) * 4u : OOB);
+                        const float pv = pring[km][c];
                         const float left_tvr = from_left(m1[c].tvr);
                         float gv = 0.0f;
                         gv += pv;
@@ -442,42 +494,45 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                             gv += left_tgd;
                             gv += cur[c].tgy;
                         }
-                        bstore(rg, chan(qo, c), gv);
-                        bstore(ry, chan(qo, c), ypp[c]);  // y of row q = t - 2
+                        const int so = (int)(qoff + (uint32_t)c * pbytes);
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), rg, (int)vst, so, 0);
+                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yq[c]), ry, (int)vst, so, 0);
                         const float ga = acc_on ? gv : 0.0f;
                         dd_add(acc[c], (double)ga * (double)ga);
                     }
                 }
             }
-#pragma unroll
-            for (int c = 0; c < C; c++) {
-                dxs1[c] = dxprev[c];
-                dys1[c] = dys[c];
-            }
         }
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            dxprev[c] = dxt[c];
-            ypp[c] = yprev[c];
-            yprev[c] = yt[c];
-        }
+    };
+
+    step(0, IC<0>{}, IC<0>{});
+    step(1, IC<1>{}, IC<1>{});
+    step(2, IC<2>{}, IC<2>{});
+    step(3, IC<0>{}, IC<3>{});
+    for (int k = 4;; k += 3) {
+        step(k, IC<1>{}, IC<4>{});
+        if (k + 1 >= kend) break;
+        step(k + 1, IC<2>{}, IC<4>{});
+        if (k + 2 >= kend) break;
+        step(k + 2, IC<0>{}, IC<4>{});
+        if (k + 3 >= kend) break;
     }
 
-    // Per-workgroup partial sums; k_finalize reduces them in a fixed order.
+    // Per-workgroup partial sums, in a fixed order: wave c merges channel c across the
+    // workgroup's waves lane by lane, then across its lanes; k_finalize reduces the
+    // workgroup partials.
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-        DD v = acc[c];
+    for (int c = 0; c < C; c++) lred[wave][c][lane] = acc[c];
+    __syncthreads();
+    if (wave < C) {
+        const int c = wave;
+        DD v = lred[0][c][lane];
+#pragma unroll
+        for (int w = 1; w < WAVES; w++) v = dd_merge(v, lred[w][c][lane]);
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) v = dd_merge(v, dd_shfl_xor(v, m));
-        if (lane == 0) red[wave][c] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned int wg = ty * gridDim.x + tx;
-#pragma unroll
-        for (int c = 0; c < C; c++) {
-            DD v = red[0][c];
-            for (int w = 1; w < WAVES; w++) v = dd_merge(v, red[w][c]);
+        if (lane == 0) {
+            const unsigned int wg = ty * gridDim.x + tx;
             a.partials[(size_t)wg * 6 + c * 2 + 0] = v.hi;
             a.partials[(size_t)wg * 6 + c * 2 + 1] = v.lo;
         }
@@ -1339,6 +1394,8 @@ int check_inputs(const artefact_params *p, const artefact_channel *ch)
     // int16 coefficients, u16 tables and a bounded grid keep every iterate far below
     // 2^60 in magnitude, so every quotient the kernels form is finite.
     if ((uint64_t)p->max_w * p->max_h > (1ull << 29)) return ARTEFACT_E_GEOMETRY;
+    // The C planes of one arena are addressed with 32-bit byte offsets below 2^31.
+    if ((uint64_t)p->nchannel * p->max_w * p->max_h * 4 >= (1ull << 31)) return ARTEFACT_E_GEOMETRY;
     if (!(p->weight >= 0.0f && p->weight < 1e6f)) return ARTEFACT_E_ARGUMENT;
     for (int c = 0; c < 3; c++)
         if (!(p->pweight[c] >= 0.0f && p->pweight[c] < 1e6f)) return ARTEFACT_E_ARGUMENT;
@@ -1397,18 +1454,14 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
     a.norms = pl->norms;
     a.rnorm = pl->rnorm;
     a.norm_sq = pl->norm_sq;
+    a.kg_rows = (uint32_t)pl->rows;
     return a;
 }
 
 template <int C, bool TGV>
 void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a)
 {
-    switch (pl->rows) {
-    case 4: hipLaunchKernelGGL((k_gradient<C, TGV, 4>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
-    case 8: hipLaunchKernelGGL((k_gradient<C, TGV, 8>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
-    case 12: hipLaunchKernelGGL((k_gradient<C, TGV, 12>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
-    default: hipLaunchKernelGGL((k_gradient<C, TGV, 16>), pl->grid_g, dim3(NT), 0, pl->stream, a); break;
-    }
+    hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, pl->stream, a);
     const unsigned int nwg = pl->grid_g.x * pl->grid_g.y;
     hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, pl->stream, (const double *)pl->partials, nwg,
                        pl->norms, pl->rnorm, pl->norm_sq);
@@ -1461,29 +1514,42 @@ int launch_iterations(artefact_hip_plan *pl)
     return hipGetLastError() == hipSuccess ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
 
-// k_gradient rows per wave.  A wave reads R + 4 rows and evaluates R + 2 rows of sources
-// for R output rows, so taller strips waste less; but the grid should not spill a
-// partial extra round over the ~4 resident waves per SIMD (1024 SIMDs).  Pick the R with
-// the smallest rounds * (R + 3.5).  An override (ARTEFACT_KG_ROWS) is honoured.
-int pick_rows(uint32_t W, uint32_t H, uint32_t C, bool tgv)
+// The k_gradient instance for (C, TGV), for occupancy queries.
+const void *gradient_kernel(uint32_t C, bool tgv)
+{
+    switch (C * 2 + (tgv ? 1 : 0)) {
+    case 2: return (const void *)k_gradient<1, false>;
+    case 3: return (const void *)k_gradient<1, true>;
+    case 4: return (const void *)k_gradient<2, false>;
+    case 5: return (const void *)k_gradient<2, true>;
+    case 6: return (const void *)k_gradient<3, false>;
+    default: return (const void *)k_gradient<3, true>;
+    }
+}
+
+// k_gradient strip height RW (output rows per wave).  A wave reads RW + 4 rows and evaluates
+// RW + 2 rows of sources, so taller strips waste less, but a grid larger than the resident
+// capacity runs a partial second round.  Pick the smallest RW whose grid (4 strips per
+// workgroup, stacked) fits the device's resident workgroups at once.  An override
+// (ARTEFACT_KG_ROWS) is honoured.
+int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv)
 {
     if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
         const int r = atoi(e);
-        if (r == 4 || r == 8 || r == 12 || (r == 16 && !(C == 3 && tgv))) return r;
+        if (r >= 1 && r <= 4096) return r;
     }
-    const uint64_t cols = (W + KG_COLS - 1) / KG_COLS, slots = 4096;
-    int best = 4;
-    double best_cost = 1e300;
-    for (int r : {4, 8, 12, 16}) {
-        if (r == 16 && C == 3 && tgv) continue;  // would exceed 128 VGPRs and spill
-        const uint64_t waves = cols * ((H + r - 1) / r);
-        const double cost = (double)((waves + slots - 1) / slots) * (r + 3.5);
-        if (cost < best_cost) {
-            best_cost = cost;
-            best = r;
-        }
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gradient_kernel(C, tgv), NT, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1 ||
+        cus < 1) {
+        per_cu = 4;
+        cus = 256;
     }
-    return best;
+    const uint64_t cols = (W + KG_COLS - 1) / KG_COLS, resident = (uint64_t)per_cu * cus;
+    // At least 2 rows: a 1-row strip spends 5 row loads per output row.
+    for (uint32_t r = 2; r < H; r++)
+        if (cols * ((H + WAVES * r - 1) / (WAVES * r)) <= resident) return (int)r;
+    return (int)std::max(H, 1u);
 }
 
 // Output stage on `stream` from device planes (y, cb, cr; cb/cr ignored unless colour) into
@@ -1573,7 +1639,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         pl->M[c] = (size_t)chs[c].rounded_w * chs[c].rounded_h;
         mtot += pl->M[c];
     }
-    pl->rows = pick_rows(pl->W, pl->H, pl->C, pl->tgv);
+    pl->rows = pick_rows(pl->device, pl->W, pl->H, pl->C, pl->tgv);
     pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * pl->rows - 1) / (WAVES * pl->rows));
     pl->grid_p = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
     const size_t nwg = (size_t)pl->grid_g.x * pl->grid_g.y;
