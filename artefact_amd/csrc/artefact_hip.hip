@@ -200,6 +200,10 @@ __device__ __forceinline__ double recip64(float b)
     return r;
 }
 
+// Norms for which recip64 is used: +normal, +subnormal or NaN (v_cmp_class mask); zero and
+// +inf take a reciprocal of 0 instead.
+__device__ __forceinline__ bool divisor_ok(float n) { return __builtin_amdgcn_class(n, 0x183); }
+
 // a / b for f32 a and b, given r ~ 1/b (see the header comment): bit-identical to IEEE.
 __device__ __forceinline__ float qdiv(float a, double r) { return (float)((double)a * r); }
 
@@ -435,13 +439,15 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                 }
                 // A zero norm or a source outside the image contributes nothing: its
                 // reciprocal is set to 0, so every contribution becomes +-0, which adds
-                // nothing to an accumulator that is never -0.
+                // nothing to an accumulator that is never -0.  An infinite norm also gets 0,
+                // which gives a / inf = +-0 for finite a and NaN for infinite a, as IEEE
+                // division does; a NaN norm keeps its NaN reciprocal.
                 const float n = sqrtf(nacc);
-                const double rn = (valid && n != 0.0f) ? recip64(n) : 0.0;
+                const double rn = (valid && divisor_ok(n)) ? recip64(n) : 0.0;
                 double rn2 = 0.0;
                 if (TGV) {
                     const float n2 = sqrtf(n2acc);
-                    rn2 = (valid && n2 != 0.0f) ? recip64(n2) : 0.0;
+                    rn2 = (valid && divisor_ok(n2)) ? recip64(n2) : 0.0;
                 }
                 Contrib *cur = slot[km];
 #pragma unroll
