@@ -200,10 +200,6 @@ __device__ __forceinline__ double recip64(float b)
     return r;
 }
 
-// Norms for which recip64 is used: +normal, +subnormal or NaN (v_cmp_class mask); zero and
-// +inf take a reciprocal of 0 instead.
-__device__ __forceinline__ bool divisor_ok(float n) { return __builtin_amdgcn_class(n, 0x183); }
-
 // a / b for f32 a and b, given r ~ 1/b (see the header comment): bit-identical to IEEE.
 __device__ __forceinline__ float qdiv(float a, double r) { return (float)((double)a * r); }
 
@@ -439,15 +435,16 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                 }
                 // A zero norm or a source outside the image contributes nothing: its
                 // reciprocal is set to 0, so every contribution becomes +-0, which adds
-                // nothing to an accumulator that is never -0.  An infinite norm also gets 0,
-                // which gives a / inf = +-0 for finite a and NaN for infinite a, as IEEE
-                // division does; a NaN norm keeps its NaN reciprocal.
+                // nothing to an accumulator that is never -0.  (An infinite norm would give
+                // recip64 = NaN where IEEE division gives a / inf = 0; it needs squared
+                // differences above 3.4e38, which int16 coefficients and u16 tables cannot
+                // produce.)
                 const float n = sqrtf(nacc);
-                const double rn = (valid && divisor_ok(n)) ? recip64(n) : 0.0;
+                const double rn = (valid && n != 0.0f) ? recip64(n) : 0.0;
                 double rn2 = 0.0;
                 if (TGV) {
                     const float n2 = sqrtf(n2acc);
-                    rn2 = (valid && divisor_ok(n2)) ? recip64(n2) : 0.0;
+                    rn2 = (valid && n2 != 0.0f) ? recip64(n2) : 0.0;
                 }
                 Contrib *cur = slot[km];
 #pragma unroll
@@ -907,6 +904,25 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
         yv[c] = *reinterpret_cast<const float4 *>(a.y[c] + p);
         gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
     }
+    // The coefficients of this thread's row-pass items, loaded now so that their latency
+    // overlaps the tile loads (the row pass runs after three barriers).
+    constexpr int ITEMS = (TOTB * 8 + NT - 1) / NT;
+    short4 cpre[ITEMS][2];
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int jb = tid + i * NT;
+        cpre[i][0] = cpre[i][1] = make_short4(0, 0, 0, 0);
+        if (jb < TOTB * 8) {
+            int bx, by;
+            const int c = fused_block<C, M0, M1, M2>(jb >> 3, x0, y0, bx, by);
+            const ChanDev &ch = a.ch[c];
+            if (bx < (int)ch.bw && by < (int)ch.bh) {
+                const size_t base = ((size_t)by * ch.bw + bx) * 64 + (jb & 7) * 8;
+                cpre[i][0] = *reinterpret_cast<const short4 *>(ch.coef + base);
+                cpre[i][1] = *reinterpret_cast<const short4 *>(ch.coef + base + 4);
+            }
+        }
+    }
     // Descent step on the extrapolated image (compute_step.rs:61-65).
     float yd[C][4];
 #pragma unroll
@@ -952,8 +968,10 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
     __syncthreads();
     // Row pass, clamp to [(c - 1/2) q, (c + 1/2) q] (compute_projection.rs:63-82) and the
     // prob difference (cos - c q) / q^2 (compute_step_prob.rs:24-30).
-    for (int jb = tid; jb < TOTB * 8; jb += NT) {
-        const int b = jb >> 3, j = jb & 7;
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        const int jb = tid + i * NT, b = jb >> 3, j = jb & 7;
+        if (jb >= TOTB * 8) break;
         int bx, by;
         const int c = fused_block<C, M0, M1, M2>(b, x0, y0, bx, by);
         const ChanDev &ch = a.ch[c];
@@ -964,8 +982,7 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
             for (int k = 0; k < 8; k++) v[k] = bp[j * 9 + k];
             dct8(v);
             const size_t base = ((size_t)by * ch.bw + bx) * 64 + j * 8;
-            const short4 c03 = *reinterpret_cast<const short4 *>(ch.coef + base);
-            const short4 c47 = *reinterpret_cast<const short4 *>(ch.coef + base + 4);
+            const short4 c03 = cpre[i][0], c47 = cpre[i][1];
             const float cf[8] = {(float)c03.x, (float)c03.y, (float)c03.z, (float)c03.w,
                                  (float)c47.x, (float)c47.y, (float)c47.z, (float)c47.w};
             float o[8], d[8];
