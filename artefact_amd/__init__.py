@@ -5,9 +5,10 @@ its resident form.  ``Coefficient`` mirrors backend/artefact-lib/jpeg/mod.rs:8-2
 ``Artefact`` mirrors the public builder of backend/artefact-lib/lib.rs:49-212.
 """
 from ._lib import ArtefactError, LIB_PATH
+from .batch import Batch, BatchImage
 from .compute import Plan, compute
 from .process import Artefact, ArtefactProcessError, ForAll, ForEach, Jpeg, JpegSource, process_coefficients
 from .types import Coefficient, max_grid
 
-__all__ = ["Artefact", "ArtefactError", "ArtefactProcessError", "Coefficient", "ForAll", "ForEach", "Jpeg",
+__all__ = ["Batch", "BatchImage", "Artefact", "ArtefactError", "ArtefactProcessError", "Coefficient", "ForAll", "ForEach", "Jpeg",
            "JpegSource", "LIB_PATH", "Plan", "compute", "max_grid", "process_coefficients"]

@@ -71,6 +71,21 @@ class artefact_process_params(ctypes.Structure):
     ]
 
 
+class artefact_tuning(ctypes.Structure):
+    _fields_ = [
+        ("kg_rows", ctypes.c_uint32),
+        ("project_kernel", ctypes.c_uint32),
+        ("ms_per_iteration", ctypes.c_float),
+    ]
+
+
+class artefact_batch_image(ctypes.Structure):
+    _fields_ = [
+        ("params", ctypes.POINTER(artefact_params)),
+        ("channels", ctypes.POINTER(artefact_channel)),
+    ]
+
+
 FLOAT_PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_float))
 U8_P = ctypes.POINTER(ctypes.c_uint8)
 
@@ -93,6 +108,15 @@ SIGNATURES = [
     ("artefact_hip_plan_download_rgb", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, U8_P]),
     ("artefact_hip_process", ctypes.c_int,
      [ctypes.POINTER(artefact_process_params), ctypes.POINTER(artefact_channel), U8_P]),
+    ("artefact_hip_plan_tune", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(artefact_tuning)]),
+    ("artefact_hip_batch_create", ctypes.c_int,
+     [ctypes.POINTER(artefact_batch_image), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+      ctypes.POINTER(ctypes.c_void_p)]),
+    ("artefact_hip_batch_run", ctypes.c_int, [ctypes.c_void_p]),
+    ("artefact_hip_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
+    ("artefact_hip_batch_download", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, FLOAT_PP]),
+    ("artefact_hip_batch_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("artefact_hip_batch_destroy", None, [ctypes.c_void_p]),
     ("artefact_hip_strerror", ctypes.c_char_p, [ctypes.c_int]),
 ]
 

@@ -137,6 +137,13 @@ class Plan:
                    "artefact_hip_plan_download_cos")
         return out
 
+    def tune(self) -> dict:
+        """Autotunes strip height and projection kernel for this geometry; re-init after."""
+        t = _lib.artefact_tuning()
+        _lib.check(self._lib.artefact_hip_plan_tune(self._h, ctypes.byref(t)), "artefact_hip_plan_tune")
+        return {"kg_rows": t.kg_rows, "project_kernel": "fused" if t.project_kernel else "per-channel",
+                "ms_per_iteration": t.ms_per_iteration}
+
     def download_rgb(self, real_w: int, real_h: int) -> np.ndarray:
         """Output stage on the GPU (lib.rs:159-211): uint8 (real_h, real_w, 3)."""
         out = np.empty((real_h, real_w, 3), dtype=np.uint8)

@@ -51,6 +51,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -1383,6 +1386,7 @@ struct artefact_hip_plan {
     bool tgv = true;
     int rows = 8;  // k_gradient rows per wave
     int layout = 0;  // LAYOUT_*: which k_project variant runs
+    int layout_fused = 0;  // the fused layout this plan's channels allow (LAYOUT_GENERIC if none)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
     ChanDev ch[3];
@@ -1482,59 +1486,74 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
 }
 
 template <int C, bool TGV>
-void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a)
+void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, pl->stream, a);
+    hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, s, a);
     const unsigned int nwg = pl->grid_g.x * pl->grid_g.y;
-    hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, pl->stream, (const double *)pl->partials, nwg,
+    hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials, nwg,
                        pl->norms, pl->rnorm, pl->norm_sq);
 }
 
 // Gradient sweep + norm: k_gradient then k_finalize.
-void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a)
+void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
     switch (pl->C * 2 + (pl->tgv ? 1 : 0)) {
-    case 2: launch_gradient_c<1, false>(pl, a); break;
-    case 3: launch_gradient_c<1, true>(pl, a); break;
-    case 4: launch_gradient_c<2, false>(pl, a); break;
-    case 5: launch_gradient_c<2, true>(pl, a); break;
-    case 6: launch_gradient_c<3, false>(pl, a); break;
-    default: launch_gradient_c<3, true>(pl, a); break;
+    case 2: launch_gradient_c<1, false>(pl, a, s); break;
+    case 3: launch_gradient_c<1, true>(pl, a, s); break;
+    case 4: launch_gradient_c<2, false>(pl, a, s); break;
+    case 5: launch_gradient_c<2, true>(pl, a, s); break;
+    case 6: launch_gradient_c<3, false>(pl, a, s); break;
+    default: launch_gradient_c<3, true>(pl, a, s); break;
     }
 }
 
-void launch_project(const artefact_hip_plan *pl, const IterArgs &a)
+void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
     switch (pl->layout) {
     case LAYOUT_Y:
-        hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, pl->stream, a);
+        hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     case LAYOUT_444:
-        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, pl->stream, a);
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     case LAYOUT_420:
-        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_SUB22, PM_SUB22>), pl->grid_p, dim3(NT), 0, pl->stream, a);
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_SUB22, PM_SUB22>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     default: break;
     }
     if (pl->C == 1)
-        hipLaunchKernelGGL(k_project<1>, pl->grid_p, dim3(NT), 0, pl->stream, a);
+        hipLaunchKernelGGL(k_project<1>, pl->grid_p, dim3(NT), 0, s, a);
     else if (pl->C == 2)
-        hipLaunchKernelGGL(k_project<2>, pl->grid_p, dim3(NT), 0, pl->stream, a);
+        hipLaunchKernelGGL(k_project<2>, pl->grid_p, dim3(NT), 0, s, a);
     else
-        hipLaunchKernelGGL(k_project<3>, pl->grid_p, dim3(NT), 0, pl->stream, a);
+        hipLaunchKernelGGL(k_project<3>, pl->grid_p, dim3(NT), 0, s, a);
 }
 
-int launch_iterations(artefact_hip_plan *pl)
+int launch_iterations(artefact_hip_plan *pl, hipStream_t s)
 {
     int cur = pl->cur;
     for (uint32_t it = 0; it < pl->iterations; it++) {
         const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == pl->iterations);
-        launch_gradient(pl, a);
-        launch_project(pl, a);
+        launch_gradient(pl, a, s);
+        launch_project(pl, a, s);
         cur = 1 - cur;
     }
     return hipGetLastError() == hipSuccess ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+// Start image from the coefficients (no override): the kernels of artefact_hip_plan_init.
+void launch_init(artefact_hip_plan *pl, hipStream_t s)
+{
+    for (uint32_t c = 0; c < pl->C; c++) {
+        const ChanDev &d = pl->ch[c];
+        float *x = pl->buf[0] + c * pl->N, *xp = pl->buf[1] + c * pl->N;
+        float *plane = pl->g + c * pl->N;  // scratch: M_c <= N
+        const size_t nb = (size_t)d.bw * d.bh;
+        hipLaunchKernelGGL(k_init_blocks, dim3((unsigned)((nb + NT / 8 - 1) / (NT / 8))), dim3(NT), 0, s, d, plane);
+        hipLaunchKernelGGL(k_init_upsample, dim3((unsigned)((pl->N + NT - 1) / NT)), dim3(NT), 0, s, d,
+                           (const float *)plane, x, xp, pl->W, pl->H);
+        hipLaunchKernelGGL(k_init_cos, dim3((unsigned)((pl->M[c] + NT - 1) / NT)), dim3(NT), 0, s, d);
+    }
 }
 
 // The k_gradient instance for (C, TGV), for occupancy queries.
@@ -1588,6 +1607,104 @@ int run_colour(hipStream_t stream, const float *y, const float *cb, const float 
     ok = ok && hipMemcpyAsync(out, staging, (size_t)real_w * real_h * 3, hipMemcpyDeviceToHost, stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(stream) == hipSuccess;
     return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+// ---- Autotuner: per geometry, the k_gradient strip height and the projection kernel ----
+// Candidates are timed on the plan itself (eager launches, HIP events around a few
+// iterations after a fresh init) and the winner is cached for the process.
+struct TuneChoice {
+    int rows, layout;
+    float ms;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune_cache;
+
+std::string tune_key(const artefact_hip_plan *pl)
+{
+    std::string k = "d" + std::to_string(pl->device) + " " + std::to_string(pl->W) + "x" + std::to_string(pl->H) +
+                    " C" + std::to_string(pl->C) + (pl->tgv ? " tgv" : " tv");
+    for (uint32_t c = 0; c < pl->C; c++)
+        k += " " + std::to_string(pl->ch[c].rw) + "x" + std::to_string(pl->ch[c].rh) + "/" +
+             std::to_string(pl->ch[c].hs) + std::to_string(pl->ch[c].vs);
+    return k;
+}
+
+void set_rows(artefact_hip_plan *pl, int rows)
+{
+    pl->rows = rows;
+    pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * rows - 1) / (WAVES * rows));
+}
+
+// Mean milliseconds per iteration over `nt` iterations after a fresh init; < 0 on failure.
+float time_loop(artefact_hip_plan *pl, uint32_t nt)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+    float ms = -1.0f;
+    if (ok) {
+        launch_init(pl, pl->stream);
+        ok = hipEventRecord(e0, pl->stream) == hipSuccess;
+        int cur = 0;
+        for (uint32_t it = 0; ok && it < nt; it++) {
+            const IterArgs a = make_args(pl, cur, pl->factors[it], false);
+            launch_gradient(pl, a, pl->stream);
+            launch_project(pl, a, pl->stream);
+            cur = 1 - cur;
+        }
+        ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, pl->stream) == hipSuccess &&
+             hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return ok ? ms / (float)nt : -1.0f;
+}
+
+void drop_graph(artefact_hip_plan *pl)
+{
+    if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
+    pl->graph = nullptr;
+}
+
+// Picks (rows, layout) for the plan, from the cache or by timing; leaves the plan
+// uninitialised.  Environment overrides (ARTEFACT_KG_ROWS, ARTEFACT_GENERIC_PROJECT) pin
+// their parameter.
+int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
+{
+    const std::string key = tune_key(pl);
+    TuneChoice best{pl->rows, pl->layout, -1.0f};
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) best = it->second;
+    }
+    if (best.ms < 0.0f && pl->iterations > 0) {
+        const int r0 = pl->rows;
+        std::vector<int> rows{r0};
+        if (!getenv("ARTEFACT_KG_ROWS"))
+            for (int r : {r0 + 1, r0 + 2, r0 - 1, r0 - 2, 2 * r0})
+                if (r >= 2 && r <= (int)pl->H && std::find(rows.begin(), rows.end(), r) == rows.end()) rows.push_back(r);
+        std::vector<int> layouts{pl->layout};
+        if (pl->layout != LAYOUT_GENERIC) layouts.push_back(LAYOUT_GENERIC);
+        const uint32_t nt = std::min<uint32_t>(std::max<uint32_t>(pl->iterations, 2), 8);
+        for (int lay : layouts)
+            for (int r : rows) {
+                pl->layout = lay;
+                set_rows(pl, r);
+                float t = time_loop(pl, nt);  // first pass warms caches and code
+                if (t < 0.0f) return ARTEFACT_E_HIP;
+                t = std::min(t, time_loop(pl, nt));
+                if (t < 0.0f) return ARTEFACT_E_HIP;
+                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, lay, t};
+            }
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tune_cache[key] = best;
+    }
+    pl->layout = best.layout;
+    set_rows(pl, best.rows);
+    drop_graph(pl);
+    pl->initialized = false;
+    if (out) *out = best;
+    return ARTEFACT_OK;
 }
 
 }  // namespace
@@ -1665,7 +1782,9 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     pl->rows = pick_rows(pl->device, pl->W, pl->H, pl->C, pl->tgv);
     pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * pl->rows - 1) / (WAVES * pl->rows));
     pl->grid_p = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
-    const size_t nwg = (size_t)pl->grid_g.x * pl->grid_g.y;
+    // Partials for the largest grid any strip height (>= 2) can give, so that the tuner can
+    // change the height without reallocating.
+    const size_t nwg = (size_t)pl->grid_g.x * ((pl->H + WAVES * 2 - 1) / (WAVES * 2));
     bool ok = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) == hipSuccess;
     ok = ok && hipMalloc(&pl->buf[0], pl->C * pl->N * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&pl->buf[1], pl->C * pl->N * sizeof(float)) == hipSuccess;
@@ -1724,6 +1843,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     else if (pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_SUB22 && pl->ch[2].mode == PM_SUB22)
         pl->layout = LAYOUT_420;
     else pl->layout = LAYOUT_GENERIC;
+    pl->layout_fused = pl->layout;
     if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
         if (atoi(e)) pl->layout = LAYOUT_GENERIC;
     ok = ok && hipMemcpyAsync(pl->quant, quant.data(), quant.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -1747,23 +1867,18 @@ int artefact_hip_plan_init(artefact_hip_plan *pl)
     if (!pl) return ARTEFACT_E_ARGUMENT;
     DeviceGuard guard(pl->device);
     bool ok = true;
-    for (uint32_t c = 0; c < pl->C; c++) {
-        const ChanDev &d = pl->ch[c];
-        float *x = pl->buf[0] + c * pl->N, *xp = pl->buf[1] + c * pl->N;
-        if (!pl->init_override.empty()) {
+    if (pl->init_override.empty()) {
+        launch_init(pl, pl->stream);
+    } else {
+        for (uint32_t c = 0; c < pl->C; c++) {
+            const ChanDev &d = pl->ch[c];
+            float *x = pl->buf[0] + c * pl->N, *xp = pl->buf[1] + c * pl->N;
             ok = ok && hipMemcpyAsync(x, pl->init_override[c].data(), pl->N * sizeof(float),
                                       hipMemcpyHostToDevice, pl->stream) == hipSuccess;
             ok = ok && hipMemcpyAsync(xp, pl->init_override[c].data(), pl->N * sizeof(float),
                                       hipMemcpyHostToDevice, pl->stream) == hipSuccess;
-        } else {
-            float *plane = pl->g + c * pl->N;  // scratch: M_c <= N
-            const size_t nb = (size_t)d.bw * d.bh;
-            hipLaunchKernelGGL(k_init_blocks, dim3((unsigned)((nb + NT / 8 - 1) / (NT / 8))), dim3(NT), 0,
-                               pl->stream, d, plane);
-            hipLaunchKernelGGL(k_init_upsample, dim3((unsigned)((pl->N + NT - 1) / NT)), dim3(NT), 0, pl->stream,
-                               d, (const float *)plane, x, xp, pl->W, pl->H);
+            hipLaunchKernelGGL(k_init_cos, dim3((unsigned)((pl->M[c] + NT - 1) / NT)), dim3(NT), 0, pl->stream, d);
         }
-        hipLaunchKernelGGL(k_init_cos, dim3((unsigned)((pl->M[c] + NT - 1) / NT)), dim3(NT), 0, pl->stream, d);
     }
     ok = ok && hipGetLastError() == hipSuccess;
     pl->cur = 0;
@@ -1783,7 +1898,7 @@ int artefact_hip_plan_iterate(artefact_hip_plan *pl)
             hipGraph_t g = nullptr;
             if (hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                 return ARTEFACT_E_HIP;
-            rc = launch_iterations(pl);
+            rc = launch_iterations(pl, pl->stream);
             const hipError_t e = hipStreamEndCapture(pl->stream, &g);
             if (rc || e != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1798,7 +1913,7 @@ int artefact_hip_plan_iterate(artefact_hip_plan *pl)
         }
         if (hipGraphLaunch(pl->graph, pl->stream) != hipSuccess) return ARTEFACT_E_HIP;
     } else {
-        rc = launch_iterations(pl);
+        rc = launch_iterations(pl, pl->stream);
         if (rc) return rc;
     }
     if (pl->iterations % 2) pl->cur = 1 - pl->cur;
@@ -1850,7 +1965,7 @@ int artefact_hip_plan_first_gradient(artefact_hip_plan *pl, float *const *out)
     if (!pl || !out) return ARTEFACT_E_ARGUMENT;
     if (!pl->initialized) return ARTEFACT_E_STATE;
     DeviceGuard guard(pl->device);
-    launch_gradient(pl, make_args(pl, pl->cur, 0.0f, false));
+    launch_gradient(pl, make_args(pl, pl->cur, 0.0f, false), pl->stream);
     bool ok = hipGetLastError() == hipSuccess;
     for (uint32_t c = 0; c < pl->C; c++) {
         if (!out[c]) return ARTEFACT_E_ARGUMENT;
@@ -1875,9 +1990,9 @@ int artefact_hip_plan_profile(artefact_hip_plan *pl, float *avg_ms)
         ok = hipEventRecord(ev[0], pl->stream) == hipSuccess;
         for (uint32_t it = 0; ok && it < n; it++) {
             const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == n);
-            launch_gradient(pl, a);
+            launch_gradient(pl, a, pl->stream);
             ok = ok && hipEventRecord(ev[2 * it + 1], pl->stream) == hipSuccess;
-            launch_project(pl, a);
+            launch_project(pl, a, pl->stream);
             ok = ok && hipEventRecord(ev[2 * it + 2], pl->stream) == hipSuccess;
             cur = 1 - cur;
         }
@@ -2033,6 +2148,168 @@ int artefact_hip_process(const artefact_process_params *p, const artefact_channe
     for (uint32_t k = 0; k < nplan; k++) artefact_hip_plan_destroy(plans[k]);
     return rc;
 }
+
+int artefact_hip_plan_tune(artefact_hip_plan *pl, artefact_tuning *out)
+{
+    if (!pl) return ARTEFACT_E_ARGUMENT;
+    DeviceGuard guard(pl->device);
+    TuneChoice t{pl->rows, pl->layout, -1.0f};
+    const int rc = tune_plan(pl, &t);
+    if (rc) return rc;
+    if (out) {
+        out->kg_rows = (uint32_t)t.rows;
+        out->project_kernel = t.layout == LAYOUT_GENERIC ? 0u : 1u;
+        out->ms_per_iteration = t.ms;
+    }
+    return ARTEFACT_OK;
+}
+
+}  // extern "C"
+
+struct artefact_hip_batch {
+    int device = 0;
+    std::vector<artefact_hip_plan *> plans;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;  // [0] fork, [k] join of stream k
+    hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+void free_batch(artefact_hip_batch *b)
+{
+    if (!b) return;
+    DeviceGuard guard(b->device);
+    for (hipStream_t s : b->streams)
+        if (s) (void)hipStreamSynchronize(s);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (hipEvent_t e : b->events)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t s : b->streams)
+        if (s) (void)hipStreamDestroy(s);
+    for (artefact_hip_plan *pl : b->plans) artefact_hip_plan_destroy(pl);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int artefact_hip_batch_create(const artefact_batch_image *images, uint32_t count, uint32_t nstreams, uint32_t tune,
+                              artefact_hip_batch **out)
+{
+    if (!out) return ARTEFACT_E_ARGUMENT;
+    *out = nullptr;
+    if (!images || !count) return ARTEFACT_E_ARGUMENT;
+    for (uint32_t i = 0; i < count; i++)
+        if (!images[i].params || !images[i].channels || images[i].params->init_override) return ARTEFACT_E_ARGUMENT;
+    artefact_hip_batch *b = new (std::nothrow) artefact_hip_batch;
+    if (!b) return ARTEFACT_E_HIP;
+    if (hipGetDevice(&b->device) != hipSuccess) {
+        delete b;
+        return ARTEFACT_E_HIP;
+    }
+    int rc = ARTEFACT_OK;
+    for (uint32_t i = 0; i < count && !rc; i++) {
+        artefact_hip_plan *pl = nullptr;
+        rc = artefact_hip_plan_create(images[i].params, images[i].channels, &pl);
+        if (!rc) {
+            pl->use_graph = false;
+            b->plans.push_back(pl);
+            if (tune) rc = tune_plan(pl, nullptr);
+        }
+    }
+    if (rc) {
+        free_batch(b);
+        return rc;
+    }
+    const uint32_t ns = std::max(1u, std::min(nstreams ? nstreams : 4u, count));
+    bool ok = true;
+    b->streams.assign(ns, nullptr);
+    b->events.assign(ns, nullptr);
+    for (uint32_t k = 0; k < ns && ok; k++) {
+        ok = hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&b->events[k], hipEventDisableTiming) == hipSuccess;
+    }
+    // Largest solves first, each to the stream with the least work so far.
+    std::vector<uint32_t> order(count), stream_of(count, 0);
+    for (uint32_t i = 0; i < count; i++) order[i] = i;
+    auto cost = [&](uint32_t i) {
+        const artefact_hip_plan *pl = b->plans[i];
+        return (double)pl->N * pl->C * (1.0 + pl->iterations);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cost(x) > cost(y); });
+    std::vector<double> load(ns, 0.0);
+    for (uint32_t i : order) {
+        const uint32_t k = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        stream_of[i] = k;
+        load[k] += cost(i);
+    }
+    // One graph: fork the streams off stream 0, every image's init and loop on its stream,
+    // join back into stream 0.
+    hipGraph_t g = nullptr;
+    ok = ok && hipStreamBeginCapture(b->streams[0], hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        bool cap = hipEventRecord(b->events[0], b->streams[0]) == hipSuccess;
+        for (uint32_t k = 1; k < ns; k++) cap = cap && hipStreamWaitEvent(b->streams[k], b->events[0], 0) == hipSuccess;
+        for (uint32_t i : order) {
+            artefact_hip_plan *pl = b->plans[i];
+            const hipStream_t s = b->streams[stream_of[i]];
+            pl->cur = 0;
+            launch_init(pl, s);
+            cap = cap && launch_iterations(pl, s) == ARTEFACT_OK;
+        }
+        for (uint32_t k = 1; k < ns; k++) {
+            cap = cap && hipEventRecord(b->events[k], b->streams[k]) == hipSuccess;
+            cap = cap && hipStreamWaitEvent(b->streams[0], b->events[k], 0) == hipSuccess;
+        }
+        const hipError_t e = hipStreamEndCapture(b->streams[0], &g);
+        ok = cap && e == hipSuccess && g;
+    }
+    ok = ok && hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) (void)hipGraphDestroy(g);
+    if (!ok) {
+        free_batch(b);
+        return ARTEFACT_E_HIP;
+    }
+    *out = b;
+    return ARTEFACT_OK;
+}
+
+int artefact_hip_batch_run(artefact_hip_batch *b)
+{
+    if (!b) return ARTEFACT_E_ARGUMENT;
+    DeviceGuard guard(b->device);
+    if (hipGraphLaunch(b->graph, b->streams[0]) != hipSuccess) return ARTEFACT_E_HIP;
+    for (artefact_hip_plan *pl : b->plans) {
+        pl->cur = (int)(pl->iterations % 2);
+        pl->initialized = false;
+    }
+    return ARTEFACT_OK;
+}
+
+int artefact_hip_batch_synchronize(artefact_hip_batch *b)
+{
+    if (!b) return ARTEFACT_E_ARGUMENT;
+    DeviceGuard guard(b->device);
+    return hipStreamSynchronize(b->streams[0]) == hipSuccess ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+int artefact_hip_batch_plan(artefact_hip_batch *b, uint32_t index, artefact_hip_plan **out)
+{
+    if (!b || !out || index >= b->plans.size()) return ARTEFACT_E_ARGUMENT;
+    *out = b->plans[index];
+    return ARTEFACT_OK;
+}
+
+int artefact_hip_batch_download(artefact_hip_batch *b, uint32_t index, float *const *out_planes)
+{
+    if (!b || index >= b->plans.size()) return ARTEFACT_E_ARGUMENT;
+    const int rc = artefact_hip_batch_synchronize(b);
+    return rc ? rc : artefact_hip_plan_download(b->plans[index], out_planes);
+}
+
+void artefact_hip_batch_destroy(artefact_hip_batch *b) { free_batch(b); }
 
 int artefact_hip_compute(const artefact_params *p, const artefact_channel *ch, float *const *out)
 {

@@ -112,6 +112,46 @@ void *artefact_hip_plan_stream(artefact_hip_plan *plan);
 int artefact_hip_plan_set_graph(artefact_hip_plan *plan, int enable);
 void artefact_hip_plan_destroy(artefact_hip_plan *plan);
 
+/* Autotuner result: the k_gradient strip height (output rows per wave) and the projection
+ * kernel (1 = the fused multi-channel kernel, 0 = the per-channel kernel) chosen for the
+ * plan's geometry, and the measured milliseconds per iteration. */
+typedef struct {
+    uint32_t kg_rows;
+    uint32_t project_kernel;
+    float ms_per_iteration;
+} artefact_tuning;
+
+/* Times candidate strip heights and projection kernels on the plan (a few iterations
+ * each) and keeps the fastest; the choice is cached per geometry for the process.  Leaves
+ * the plan uninitialised (call artefact_hip_plan_init next).  ARTEFACT_KG_ROWS and
+ * ARTEFACT_GENERIC_PROJECT in the environment pin their parameter.  No reference
+ * counterpart: the reference has one fixed schedule. */
+int artefact_hip_plan_tune(artefact_hip_plan *plan, artefact_tuning *out);
+
+/* Many images in one hipGraph (the batched form).  Each image is one compute() call's
+ * arguments (init_override must be NULL).  The graph holds every image's start image and
+ * iteration loop; images are spread over `streams` HIP streams (0: 4), largest first onto
+ * the least loaded, so independent solves overlap on the GPU.  With `tune` set every
+ * geometry is autotuned first.  No reference counterpart: the reference processes one
+ * image per call (lib.rs:99-157). */
+typedef struct {
+    const artefact_params *params;
+    const artefact_channel *channels;
+} artefact_batch_image;
+typedef struct artefact_hip_batch artefact_hip_batch;
+
+int artefact_hip_batch_create(const artefact_batch_image *images, uint32_t count, uint32_t streams, uint32_t tune,
+                              artefact_hip_batch **out_batch);
+/* Launches the graph (asynchronous): every image is solved from its coefficients. */
+int artefact_hip_batch_run(artefact_hip_batch *batch);
+int artefact_hip_batch_synchronize(artefact_hip_batch *batch);
+/* Synchronises and copies image `index`'s planes out, as artefact_hip_plan_download. */
+int artefact_hip_batch_download(artefact_hip_batch *batch, uint32_t index, float *const *out_planes);
+/* The plan of image `index` (owned by the batch), for the plan calls that read results
+ * (download_rgb, objective, last_norms); synchronise the batch first. */
+int artefact_hip_batch_plan(artefact_hip_batch *batch, uint32_t index, artefact_hip_plan **out_plan);
+void artefact_hip_batch_destroy(artefact_hip_batch *batch);
+
 /* Output stage of Artefact::process (lib.rs:159-211), run on the GPU from the plan's
  * current planes: luma + 128, then for three channels R = 1.402*cr + y,
  * G = 0.71414*(-cr) + (0.34414*(-cb) + y), B = 1.772*cb + y with every product and sum
