@@ -8,7 +8,8 @@ from ._lib import ArtefactError, LIB_PATH
 from .batch import Batch, BatchImage
 from .compute import Plan, compute
 from .process import Artefact, ArtefactProcessError, ForAll, ForEach, Jpeg, JpegSource, process_coefficients
+from .jpeg import JpegDecodeError, decode as decode_jpeg
 from .types import Coefficient, max_grid
 
 __all__ = ["Batch", "BatchImage", "Artefact", "ArtefactError", "ArtefactProcessError", "Coefficient", "ForAll", "ForEach", "Jpeg",
-           "JpegSource", "LIB_PATH", "Plan", "compute", "max_grid", "process_coefficients"]
+           "JpegSource", "JpegDecodeError", "decode_jpeg", "LIB_PATH", "Plan", "compute", "max_grid", "process_coefficients"]

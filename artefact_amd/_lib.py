@@ -21,6 +21,7 @@ ERRORS = {
     -3: "zero entry in a quantisation table",
     -4: "HIP runtime failure",
     -5: "plan used out of order",
+    -6: "malformed or unsupported JPEG data",
     1: "benchmark run: the loop ran and no image was written",
 }
 ARTEFACT_BENCHMARK = 1
@@ -117,6 +118,13 @@ SIGNATURES = [
     ("artefact_hip_batch_download", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, FLOAT_PP]),
     ("artefact_hip_batch_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
     ("artefact_hip_batch_destroy", None, [ctypes.c_void_p]),
+    ("artefact_jpeg_decode", ctypes.c_int,
+     [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]),
+    ("artefact_jpeg_nchannel", ctypes.c_uint32, [ctypes.c_void_p]),
+    ("artefact_jpeg_size", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    ("artefact_jpeg_channels", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(artefact_channel)]),
+    ("artefact_jpeg_free", None, [ctypes.c_void_p]),
     ("artefact_hip_strerror", ctypes.c_char_p, [ctypes.c_int]),
 ]
 

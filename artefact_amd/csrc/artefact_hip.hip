@@ -1720,6 +1720,8 @@ const char *artefact_hip_strerror(int code)
     case ARTEFACT_E_QUANT: return "zero entry in a quantisation table";
     case ARTEFACT_E_HIP: return "HIP runtime failure";
     case ARTEFACT_E_STATE: return "plan used out of order";
+    case ARTEFACT_E_JPEG: return "malformed or unsupported JPEG data";
+    case ARTEFACT_BENCHMARK: return "benchmark run: no image written";
     default: return "unknown error";
     }
 }

@@ -19,6 +19,7 @@
 #ifndef ARTEFACT_HIP_H
 #define ARTEFACT_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -32,6 +33,7 @@ extern "C" {
 #define ARTEFACT_E_QUANT        -3  /* a quantisation table entry is 0 (the reference divides by q^2, compute_step_prob.rs:29) */
 #define ARTEFACT_E_HIP          -4  /* HIP runtime failure (allocation, launch, copy) */
 #define ARTEFACT_E_STATE        -5  /* plan used out of order (iterate before init) */
+#define ARTEFACT_E_JPEG         -6  /* malformed or unsupported JPEG data (artefact_jpeg_decode) */
 #define ARTEFACT_BENCHMARK       1  /* artefact_hip_process with benchmark set: the loop ran, no image was
                                        written (the reference returns Err("BENCHMARK"), lib.rs:155-157) */
 
@@ -185,6 +187,23 @@ typedef struct {
  * benchmark is set). */
 int artefact_hip_process(const artefact_process_params *params, const artefact_channel *channels,
                          uint8_t *out_rgb);
+
+/* ---- JPEG input: Jpeg::from (backend/artefact-lib/jpeg/zune.rs:5-62) on the host ----
+ * Decodes baseline/extended sequential and progressive Huffman JPEGs (8-bit, 1-4 components,
+ * sampling factors 1 or 2) to the reference's Coefficient layout: per component the plane
+ * rounded to the MCU, int16 coefficients in raster block order and natural order inside a
+ * block, not dequantised, and the component's quantisation table in natural order.  The
+ * zune-jpeg fork's placement rules are kept (see jpeg_decode.cpp).  On failure returns
+ * ARTEFACT_E_JPEG and writes a message into err (err_len bytes, may be NULL). */
+typedef struct artefact_jpeg artefact_jpeg;
+int artefact_jpeg_decode(const uint8_t *data, size_t size, artefact_jpeg **out_jpeg, char *err, size_t err_len);
+/* Jpeg::nchannel, real_px_w, real_px_h (jpeg/mod.rs:29-36). */
+uint32_t artefact_jpeg_nchannel(const artefact_jpeg *jpeg);
+int artefact_jpeg_size(const artefact_jpeg *jpeg, uint32_t *real_w, uint32_t *real_h);
+/* Fills nchannel artefact_channel entries whose pointers stay valid until artefact_jpeg_free;
+ * they can go straight to artefact_hip_process or artefact_hip_compute. */
+int artefact_jpeg_channels(const artefact_jpeg *jpeg, artefact_channel *out_channels);
+void artefact_jpeg_free(artefact_jpeg *jpeg);
 
 const char *artefact_hip_strerror(int code);
 

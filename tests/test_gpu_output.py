@@ -71,3 +71,23 @@ def test_artefact_builder_defaults_and_benchmark():
         a.process()
     with pytest.raises(artefact_amd.ArtefactProcessError, match="Source is not set"):
         artefact_amd.Artefact().process()
+
+
+@pytest.mark.parametrize("sub,prog,sep", [(2, False, False), (0, True, True), (1, False, False)])
+def test_process_from_jpeg_bytes(sub, prog, sep):
+    """Artefact::process end to end: JPEG bytes (written by libjpeg through Pillow) ->
+    host decoder -> GPU loop(s) -> GPU output stage, byte-exact against the oracle run on
+    the decoded coefficients."""
+    import io
+    from PIL import Image
+    rgb = np.stack(synth.synth_rgb(45, 70, 12), -1)
+    f = io.BytesIO()
+    Image.fromarray(rgb).save(f, "JPEG", quality=60, subsampling=sub, progressive=prog)
+    data = f.getvalue()
+    print(f"case jpeg sub={sub} progressive={prog} separate={sep}", flush=True)
+    img = (artefact_amd.Artefact().source(artefact_amd.JpegSource.Buffer(data))
+           .iterations(artefact_amd.ForAll(4)).separate_components(sep).process())
+    j = artefact_amd.decode_jpeg(data)
+    ref = oracle.process(j.coefs, 70, 45, [0.3] * 3, [0.001] * 3, [4] * 3, sep, norm_mode="f64")
+    assert img.shape == (45, 70, 3)
+    assert np.array_equal(img, ref)

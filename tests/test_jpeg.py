@@ -1,0 +1,165 @@
+"""JPEG input stage (host C++ decoder in the library) against independent references.
+
+* libjpeg through Pillow: luma (and 4:4:4 chroma) rebuilt from the decoded coefficients
+  (dequantise, orthonormal IDCT in f64, +128, round) is within 1 of libjpeg's own decoded
+  samples, for baseline and progressive files at several qualities and subsamplings, with
+  and without restart markers; quantisation tables equal the IJG tables scaled to the
+  quality, in natural order.
+* progressive == baseline: libjpeg quantises once, so the progressive and the sequential
+  file of the same image and quality hold identical coefficients.
+* a T.81 writer (tests/jpeg_writer.py) over known coefficients: the decoder returns them
+  exactly, including the fork's block placement for a (h=1, v=2) component.
+No GPU is needed: decoding is host code.
+"""
+import io
+
+import numpy as np
+import pytest
+from PIL import Image
+
+import artefact_amd
+from artefact_amd import synth
+from jpeg_writer import write_baseline
+
+D = synth.dct_matrix()
+
+
+def rebuild(c):
+    b = c.dct_coefs.reshape(c.block_h, c.block_w, 8, 8).astype(np.float64) * c.quant_table.reshape(8, 8)
+    px = np.einsum("ux,abuv,vy->abxy", D, b, D) + 128.0
+    return px.transpose(0, 2, 1, 3).reshape(c.rounded_px_h, c.rounded_px_w)
+
+
+def pil_jpeg(img, **kw):
+    f = io.BytesIO()
+    Image.fromarray(img).save(f, "JPEG", **kw)
+    return f.getvalue()
+
+
+def pil_ycc(data):
+    im = Image.open(io.BytesIO(data))
+    im.draft("YCbCr" if im.mode != "L" else "L", im.size)
+    im.load()
+    a = np.asarray(im).astype(np.float64)
+    return a[..., None] if a.ndim == 2 else a
+
+
+CASES = [(sub, prog, q, rst) for sub in (0, 1, 2) for prog in (False, True) for q in (30, 75, 95)
+         for rst in (None,)] + [(2, False, 75, 2), (0, False, 50, 5), (1, True, 75, 3)]
+
+
+@pytest.mark.parametrize("sub,prog,quality,rst", CASES)
+def test_matches_libjpeg_samples(sub, prog, quality, rst):
+    h, w = 61, 93
+    rgb = np.stack(synth.synth_rgb(h, w, quality + sub), -1)
+    kw = dict(quality=quality, subsampling=sub, progressive=prog)
+    if rst:
+        kw["restart_marker_blocks"] = rst
+    data = pil_jpeg(rgb, **kw)
+    j = artefact_amd.decode_jpeg(data)
+    assert (j.nchannel, j.real_px_w, j.real_px_h) == (3, w, h)
+    hs, vs = {0: (1, 1), 1: (2, 1), 2: (2, 2)}[sub]
+    assert [(c.horizontal_samp_factor, c.vertical_samp_factor) for c in j.coefs] == [(1, 1), (hs, vs), (hs, vs)]
+    W = -(-w // (8 * hs)) * 8 * hs
+    H = -(-h // (8 * vs)) * 8 * vs
+    assert (j.coefs[0].rounded_px_w, j.coefs[0].rounded_px_h) == (W, H)
+    assert (j.coefs[1].rounded_px_w, j.coefs[1].rounded_px_h) == (W // hs, H // vs)
+    assert np.array_equal(j.coefs[0].quant_table, synth.quant_table(quality))
+    assert np.array_equal(j.coefs[1].quant_table, synth.quant_table(quality, chroma=True))
+    ycc = pil_ycc(data)
+    planes = [0, 1, 2] if sub == 0 else [0]
+    for p in planes:
+        mine = np.clip(np.round(rebuild(j.coefs[p])[:h, :w]), 0, 255)
+        assert np.abs(mine - ycc[..., p]).max() <= 1, f"plane {p}"
+
+
+@pytest.mark.parametrize("sub", [0, 1, 2])
+def test_progressive_equals_sequential(sub):
+    rgb = np.stack(synth.synth_rgb(80, 72, 11), -1)
+    a = artefact_amd.decode_jpeg(pil_jpeg(rgb, quality=80, subsampling=sub))
+    b = artefact_amd.decode_jpeg(pil_jpeg(rgb, quality=80, subsampling=sub, progressive=True))
+    for x, y in zip(a.coefs, b.coefs):
+        assert np.array_equal(x.dct_coefs, y.dct_coefs)
+
+
+@pytest.mark.parametrize("prog", [False, True])
+def test_grey(prog):
+    g = synth.synth_plane(37, 50, 4)
+    data = pil_jpeg(g, quality=60, progressive=prog)
+    j = artefact_amd.decode_jpeg(data)
+    assert (j.nchannel, j.real_px_w, j.real_px_h) == (1, 50, 37)
+    assert (j.coefs[0].rounded_px_w, j.coefs[0].rounded_px_h) == (56, 40)
+    mine = np.clip(np.round(rebuild(j.coefs[0])[:37, :50]), 0, 255)
+    assert np.abs(mine - pil_ycc(data)[..., 0]).max() <= 1
+
+
+def _std_components(coefs, jpeg_hv):
+    out = []
+    for c, (h, v) in zip(coefs, jpeg_hv):
+        out.append((h, v, c.quant_table, c.dct_coefs, c.block_w))
+    return out
+
+
+@pytest.mark.parametrize("sub,rst", [("444", 0), ("420", 0), ("422", 0), ("420", 3), ("444", 7)])
+def test_writer_round_trip_exact(sub, rst):
+    w, h = 88, 56
+    coefs = synth.encode(synth.synth_rgb(h, w, 5), subsampling=sub, quality=70)
+    hv = {"444": [(1, 1)] * 3, "420": [(2, 2), (1, 1), (1, 1)], "422": [(2, 1), (1, 1), (1, 1)]}[sub]
+    data = write_baseline(w, h, _std_components(coefs, hv), restart_interval=rst)
+    j = artefact_amd.decode_jpeg(data)
+    for a, b in zip(j.coefs, coefs):
+        assert (a.rounded_px_w, a.rounded_px_h, a.horizontal_samp_factor, a.vertical_samp_factor) == \
+               (b.rounded_px_w, b.rounded_px_h, b.horizontal_samp_factor, b.vertical_samp_factor)
+        assert np.array_equal(a.dct_coefs, b.dct_coefs)
+        assert np.array_equal(a.quant_table, b.quant_table)
+
+
+def test_one_by_two_component_keeps_the_fork_placement():
+    """A (h=1, v=2) luma: the fork stores an MCU's two blocks at 2m and 2m+1 (mcu.rs:235-249)."""
+    w, h = 40, 32
+    coefs = synth.encode(synth.synth_rgb(h, w, 6), subsampling="440", quality=70)
+    data = write_baseline(w, h, _std_components(coefs, [(1, 2), (1, 1), (1, 1)]))
+    j = artefact_amd.decode_jpeg(data)
+    y = coefs[0]
+    mw, mh = y.block_w, y.block_h // 2
+    std = y.dct_coefs.reshape(-1, 64)
+    want = np.empty_like(std)
+    for r in range(mh):
+        for c in range(mw):
+            m = r * mw + c
+            want[2 * m] = std[2 * r * mw + c]
+            want[2 * m + 1] = std[(2 * r + 1) * mw + c]
+    assert np.array_equal(j.coefs[0].dct_coefs.reshape(-1, 64), want)
+    for a, b in zip(j.coefs[1:], coefs[1:]):
+        assert np.array_equal(a.dct_coefs, b.dct_coefs)
+
+
+def test_truncated_entropy_data_keeps_zeros():
+    rgb = np.stack(synth.synth_rgb(128, 128, 8), -1)
+    data = pil_jpeg(rgb, quality=75, subsampling=2)
+    full = artefact_amd.decode_jpeg(data)
+    cut = artefact_amd.decode_jpeg(data[: len(data) * 2 // 3])
+    # The first block rows agree; the missing tail decodes to zeros or stops early.
+    assert np.array_equal(cut.coefs[0].dct_coefs[: 64 * 32], full.coefs[0].dct_coefs[: 64 * 32])
+    assert not np.array_equal(cut.coefs[0].dct_coefs, full.coefs[0].dct_coefs)
+
+
+@pytest.mark.parametrize("data", [b"", b"not a jpeg at all", b"\xff\xd8\xff\xd9", b"\xff\xd8\xff\xdb\x00"])
+def test_malformed_data_raises(data):
+    with pytest.raises(artefact_amd.JpegDecodeError):
+        artefact_amd.decode_jpeg(data)
+
+
+def test_unsupported_process_raises():
+    data = bytearray(pil_jpeg(np.zeros((16, 16), np.uint8), quality=90))
+    i = data.index(b"\xff\xc0")
+    data[i + 1] = 0xC3  # lossless
+    with pytest.raises(artefact_amd.JpegDecodeError, match="unsupported"):
+        artefact_amd.decode_jpeg(bytes(data))
+
+
+def test_source_errors_use_the_reference_messages(tmp_path):
+    with pytest.raises(artefact_amd.ArtefactProcessError, match="Failed to read JPEG: Failed to read JPEG file"):
+        artefact_amd.Artefact().source(artefact_amd.JpegSource.File(str(tmp_path / "missing.jpg"))).process()
+    with pytest.raises(artefact_amd.ArtefactProcessError, match="Failed to read JPEG: Failed to decode JPEG"):
+        artefact_amd.Artefact().source(artefact_amd.JpegSource.Buffer(b"garbage")).process()
