@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libartefact_hip.so")
 
 ARTEFACT_MODE_TGV = 0
 ARTEFACT_MODE_TV = 1
+ARTEFACT_PRECISION_F32 = 0
+ARTEFACT_PRECISION_BF16X3 = 1
 
 ERRORS = {
     -1: "invalid argument",
@@ -110,6 +112,7 @@ SIGNATURES = [
     ("artefact_hip_process", ctypes.c_int,
      [ctypes.POINTER(artefact_process_params), ctypes.POINTER(artefact_channel), U8_P]),
     ("artefact_hip_plan_tune", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(artefact_tuning)]),
+    ("artefact_hip_plan_set_precision", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("artefact_hip_batch_create", ctypes.c_int,
      [ctypes.POINTER(artefact_batch_image), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
       ctypes.POINTER(ctypes.c_void_p)]),

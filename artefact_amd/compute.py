@@ -137,6 +137,11 @@ class Plan:
                    "artefact_hip_plan_download_cos")
         return out
 
+    def set_precision(self, precision: str) -> None:
+        """"f32" (default, the reference's arithmetic) or "bf16x3" (MFMA projection transforms)."""
+        code = {"f32": _lib.ARTEFACT_PRECISION_F32, "bf16x3": _lib.ARTEFACT_PRECISION_BF16X3}[precision]
+        _lib.check(self._lib.artefact_hip_plan_set_precision(self._h, code), "artefact_hip_plan_set_precision")
+
     def tune(self) -> dict:
         """Autotunes strip height and projection kernel for this geometry; re-init after."""
         t = _lib.artefact_tuning()

@@ -1374,6 +1374,129 @@ __global__ __launch_bounds__(NT) void k_colour(ColourArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Precision form of the projection: the 8x8 DCT and IDCTs as BF16 MFMA contractions with
+// split hi/lo operands and f32 accumulation (v_mfma_f32_16x16x16_bf16), for layouts whose
+// channels all cover the grid at full resolution (4:4:4, luma only).  Not bit-identical to
+// the reference: each product keeps about 16 significant bits.
+//
+// A wave owns a 16x16 super-tile (2x2 blocks) and holds it in the MFMA accumulator layout:
+// lane l, item t is element (row 4*(l/16)+t, column l%16).  With Db = blockdiag(D, D) (D the
+// orthonormal 8-point DCT-II matrix), one stage computes C = A * B with B a constant and A
+// the TRANSPOSE of the previous stage's result, because the A operand layout (lane l holds
+// A[l%16][4*(l/16)+t]) is the accumulator layout transposed.  So two chained stages give
+//   forward:  Y = (X^T Db^T)^T Db^T = Db X Db^T,    inverse:  X = (Y^T Db)^T Db = Db^T Y Db
+// per 8x8 block, with no shuffles, no LDS and no barriers.  Each stage is three MFMAs:
+// a_lo*b_hi + a_hi*b_lo + a_hi*b_hi (the a_lo*b_lo term, ~2^-16 relative, is dropped).
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_rne(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(short h) { return __uint_as_float((uint32_t)(uint16_t)h << 16); }
+
+struct SplitB {
+    bf16x4 hi, lo;
+};
+__device__ __forceinline__ f32x4 mfma_stage(const f32x4 a, const SplitB &b)
+{
+    bf16x4 ah, al;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        ah[t] = bf16_rne(a[t]);
+        al[t] = bf16_rne(a[t] - bf16_to_f32(ah[t]));
+    }
+    f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b.hi, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b.lo, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b.hi, c, 0, 0, 0);
+    return c;
+}
+
+template <int C>
+__global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int W = a.W, H = a.H;
+    const int x0 = blockIdx.x * TW + wave * 16, y0 = blockIdx.y * TH;
+    const int col = lane & 15, rq = (lane >> 4) * 4;  // element (rq + t, col) of the super-tile
+    // B operands: element (k = rq + t, n = col).  Forward stages use Db^T, inverse stages Db.
+    SplitB fwd, inv;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int k = rq + t, n = col;
+        const bool same = (k >> 3) == (n >> 3);
+        const int ku = k & 7, nu = n & 7;
+        auto dct = [](int u, int x) -> float {
+            const double cu = u == 0 ? 0.35355339059327376220 : 0.5;
+            return (float)(cu * cos((2 * x + 1) * u * 3.14159265358979323846 / 16.0));
+        };
+        const float vf = same ? dct(nu, ku) : 0.0f;  // Db^T[k][n] = D[n][k]
+        const float vi = same ? dct(ku, nu) : 0.0f;  // Db[k][n] = D[k][n]
+        fwd.hi[t] = bf16_rne(vf);
+        fwd.lo[t] = bf16_rne(vf - bf16_to_f32(fwd.hi[t]));
+        inv.hi[t] = bf16_rne(vi);
+        inv.lo[t] = bf16_rne(vi - bf16_to_f32(inv.hi[t]));
+    }
+    const int px = x0 + col;
+    const bool xin = px < W;
+    const float step = a.step;
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        const ChanDev &ch = a.ch[c];
+        const float norm = a.norms[c];
+        const double rn = a.rnorm[c];
+        f32x4 xv;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int py = y0 + rq + t;
+            float v = 0.0f;
+            if (xin && py < H) {
+                const size_t p = (size_t)py * W + px;
+                v = a.y[c][p];
+                // Descent step on the extrapolated image (compute_step.rs:61-65).
+                if (norm != 0.0f) v -= step * qdiv(a.g[c][p], rn);
+            }
+            xv[t] = v;
+        }
+        // Forward DCT, clamp to [(c - 1/2) q, (c + 1/2) q], prob difference (cos - c q) / q^2.
+        const f32x4 yv = mfma_stage(mfma_stage(xv, fwd), fwd);
+        f32x4 ov, dv;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int py = y0 + rq + t;
+            const int k = ((rq + t) & 7) * 8 + (col & 7);
+            float o = 0.0f, d = 0.0f;
+            if (xin && py < H) {
+                const size_t base = ((size_t)(py >> 3) * ch.bw + (px >> 3)) * 64;
+                const float q = ch.quant[k];
+                const float cf = (float)ch.coef[base + k];
+                o = clamp_ref(yv[t], (cf - 0.5f) * q, (cf + 0.5f) * q);
+                d = qdiv(o - cf * q, ch.rqq[k]);
+                if (a.write_cos) ch.cos[base + k] = o;
+            }
+            ov[t] = o;
+            dv[t] = d;
+        }
+        const f32x4 xn = mfma_stage(mfma_stage(ov, inv), inv);
+        const bool prob_on = ch.alpha_p != 0.0f;
+        f32x4 pn = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (prob_on) pn = mfma_stage(mfma_stage(dv, inv), inv);
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int py = y0 + rq + t;
+            if (xin && py < H) {
+                const size_t p = (size_t)py * W + px;
+                a.xnew[c][p] = xn[t];
+                if (prob_on) ch.prob[(size_t)py * ch.rw + px] = ch.alpha_p * pn[t];
+            }
+        }
+    }
+}
+
 }  // namespace
 
 struct artefact_hip_plan {
@@ -1387,6 +1510,7 @@ struct artefact_hip_plan {
     int rows = 8;  // k_gradient rows per wave
     int layout = 0;  // LAYOUT_*: which k_project variant runs
     int layout_fused = 0;  // the fused layout this plan's channels allow (LAYOUT_GENERIC if none)
+    uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
     ChanDev ch[3];
@@ -1509,6 +1633,12 @@ void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t
 
 void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
+    if (pl->precision == ARTEFACT_PRECISION_BF16X3) {
+        if (pl->C == 1) hipLaunchKernelGGL(k_project_mfma<1>, pl->grid_p, dim3(NT), 0, s, a);
+        else if (pl->C == 2) hipLaunchKernelGGL(k_project_mfma<2>, pl->grid_p, dim3(NT), 0, s, a);
+        else hipLaunchKernelGGL(k_project_mfma<3>, pl->grid_p, dim3(NT), 0, s, a);
+        return;
+    }
     switch (pl->layout) {
     case LAYOUT_Y:
         hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
@@ -2149,6 +2279,22 @@ int artefact_hip_process(const artefact_process_params *p, const artefact_channe
     }
     for (uint32_t k = 0; k < nplan; k++) artefact_hip_plan_destroy(plans[k]);
     return rc;
+}
+
+int artefact_hip_plan_set_precision(artefact_hip_plan *pl, uint32_t precision)
+{
+    if (!pl) return ARTEFACT_E_ARGUMENT;
+    if (precision != ARTEFACT_PRECISION_F32 && precision != ARTEFACT_PRECISION_BF16X3) return ARTEFACT_E_ARGUMENT;
+    if (precision == ARTEFACT_PRECISION_BF16X3)
+        for (uint32_t c = 0; c < pl->C; c++)
+            if (pl->ch[c].mode != PM_FULL) return ARTEFACT_E_GEOMETRY;
+    if (pl->precision != precision) {
+        DeviceGuard guard(pl->device);
+        if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+        drop_graph(pl);
+    }
+    pl->precision = precision;
+    return ARTEFACT_OK;
 }
 
 int artefact_hip_plan_tune(artefact_hip_plan *pl, artefact_tuning *out)

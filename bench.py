@@ -137,6 +137,86 @@ def cpu_baseline(wl, iters):
     }
 
 
+def bench_precision(args) -> int:
+    """Precision form on the 4:4:4 4096x3072 q30 TGV workload: the projection's transforms on
+    BF16 MFMA (hi/lo, f32 accumulation); error of the result against the FP64 oracle loop per
+    channel, next to the plain f32 path's."""
+    import numpy as np
+
+    import artefact_amd
+    import oracle
+    from artefact_amd import synth
+
+    wl = synth.workload("tgv_444_4096", seed=0)
+    W, H, iters = wl["max_w"], wl["max_h"], wl["iterations"]
+    coefs = wl["coefs"]
+    print(f"bench: precision form {W}x{H} iters={iters}", file=sys.stderr, flush=True)
+    res_by = {}
+    outs = {}
+    for prec in ("f32", "bf16x3"):
+        plan = artefact_amd.Plan(coefs, wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"], graph=True)
+        plan.set_precision(prec)
+        for _ in range(args.warmup):
+            plan.init()
+            plan.iterate()
+        plan.download()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.init()
+            plan.iterate()
+        plan.last_norms()
+        ms = 1e3 * (time.perf_counter() - t0) / args.steps
+        plan.init()
+        prof = plan.profile()
+        plan.init()
+        plan.iterate()
+        obj, _ = plan.objective()
+        outs[prec] = plan.download()
+        plan.close()
+        res_by[prec] = {"ms_per_step": round(ms, 4), "k_project_ms": round(prof["k_project_ms"], 5),
+                        "k_gradient_ms": round(prof["k_gradient_ms"], 5), "objective": obj}
+        print(f"bench: {prec} {ms:.3f} ms/solve", file=sys.stderr, flush=True)
+    errors = None
+    if not args.no_error:
+        t0 = time.perf_counter()
+        ref = oracle.compute_f64(coefs, wl["weight"], wl["pweight"], iters, W, H)
+        fref, _ = oracle.objective(coefs, wl["weight"], wl["pweight"], W, H, ref)
+        errors = {"fp64_oracle_s": round(time.perf_counter() - t0, 1)}
+        for prec, out in outs.items():
+            e = [np.abs(o.astype(np.float64) - r) for o, r in zip(out, ref)]
+            errors[prec] = {
+                "max_abs_per_channel": [round(float(x.max()), 5) for x in e],
+                "rms_per_channel": [float(f"{np.sqrt((x ** 2).mean()):.4g}") for x in e],
+                "rel_objective_error": float(f"{(res_by[prec]['objective'] - fref) / fref:.3g}"),
+            }
+    ms = res_by["bf16x3"]["ms_per_step"]
+    B = algorithmic_bytes(coefs, W, H)
+    t_iter_ms = res_by["bf16x3"]["k_gradient_ms"] + res_by["bf16x3"]["k_project_ms"]
+    achieved = B / (t_iter_ms * 1e-3) / 1e9
+    res = {
+        "metric": "Mpixel*iterations/s + time to plain-100-iter objective",
+        "value": round(W * H * iters / (ms * 1e-3) / 1e6, 2),
+        "unit": "Mpixel*it/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 transforms, f32 stencil, f64 objective",
+        "data": "synthetic (seeded generator, artefact_amd/synth.py)",
+        "config": {
+            "workload": "precision form: " + describe(wl) + "; projection DCT/IDCTs on BF16 MFMA with hi/lo split",
+            "by_precision": res_by,
+            "error_vs_fp64_oracle": errors,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "one iteration = k_gradient + k_project_mfma",
+            "algorithmic_bytes_per_iteration": B,
+        },
+    }
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def bench_batch(args) -> int:
     """The batched form: ``batchN`` = N images (synth.batch_workload) in one graph."""
     import artefact_amd
@@ -230,6 +310,7 @@ def main() -> int:
     ap.add_argument("--workload", default="tgv_420_2048")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true", help="precision workload: skip the FP64 oracle run")
     ap.add_argument("--copy-sweep", action="store_true", help="also time 16/64/128 MiB copies")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print per-kernel HBM bytes from two rocprofv3 --pmc CSVs and exit")
@@ -249,6 +330,8 @@ def main() -> int:
 
     if args.workload.startswith("batch"):
         return bench_batch(args)
+    if args.workload == "precision":
+        return bench_precision(args)
 
     wl = synth.workload(args.workload, seed=0)
     W, H, iters = wl["max_w"], wl["max_h"], wl["iterations"]

@@ -40,6 +40,9 @@ extern "C" {
 #define ARTEFACT_MODE_TGV 0  /* TV + second-order term with strength weight (compute_step.rs:42-48) */
 #define ARTEFACT_MODE_TV  1  /* TV only: the same loop with weight forced to 0 */
 
+#define ARTEFACT_PRECISION_F32    0  /* the reference's f32 arithmetic, bit for bit (default) */
+#define ARTEFACT_PRECISION_BF16X3 1  /* projection DCT/IDCTs on BF16 MFMA, hi/lo split, f32 accumulation */
+
 /* One colour component: backend/artefact-lib/jpeg/mod.rs:8-27 (Coefficient).
  * The reference carries coefficients and the quantisation table as f32 holding
  * i16 / u16 values (backend/artefact-lib/jpeg/zune.rs:33-45); they pass through
@@ -113,6 +116,13 @@ void *artefact_hip_plan_stream(artefact_hip_plan *plan);
 /* Enables (1, default) or disables (0) capturing the iteration loop in a hipGraph. */
 int artefact_hip_plan_set_graph(artefact_hip_plan *plan, int enable);
 void artefact_hip_plan_destroy(artefact_hip_plan *plan);
+
+/* Selects the projection's arithmetic: ARTEFACT_PRECISION_F32 (default, bit-identical to the
+ * reference) or ARTEFACT_PRECISION_BF16X3 (the precision form: every 8x8 DCT and IDCT of the
+ * projection as BF16 MFMA contractions with hi/lo operands and f32 accumulation; the stencil
+ * and descent stay f32).  BF16X3 needs every channel at full grid resolution (4:4:4 or one
+ * channel), else ARTEFACT_E_GEOMETRY.  No reference counterpart. */
+int artefact_hip_plan_set_precision(artefact_hip_plan *plan, uint32_t precision);
 
 /* Autotuner result: the k_gradient strip height (output rows per wave) and the projection
  * kernel (1 = the fused multi-channel kernel, 0 = the per-channel kernel) chosen for the
