@@ -121,6 +121,24 @@ SIGNATURES = [
     ("artefact_hip_batch_download", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, FLOAT_PP]),
     ("artefact_hip_batch_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),
     ("artefact_hip_batch_destroy", None, [ctypes.c_void_p]),
+    ("artefact_cnn_param_count", ctypes.c_uint32, []),
+    ("artefact_cnn_create", ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_void_p)]),
+    ("artefact_cnn_get_params", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("artefact_cnn_forward", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_uint32,
+      ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]),
+    ("artefact_cnn_gradients", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+      ctypes.POINTER(ctypes.c_float)]),
+    ("artefact_cnn_set_dataset", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("artefact_cnn_train_step", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_float,
+      ctypes.POINTER(ctypes.c_double)]),
+    ("artefact_cnn_destroy", None, [ctypes.c_void_p]),
+    ("artefact_hip_plan_warm_init", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("artefact_jpeg_decode", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]),
     ("artefact_jpeg_nchannel", ctypes.c_uint32, [ctypes.c_void_p]),

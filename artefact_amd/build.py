@@ -1,7 +1,7 @@
 """Builds libartefact_hip.so in-tree with hipcc for gfx950.
 
-Sources: the HIP kernels and their C ABI (csrc/artefact_hip.hip) and the host-side JPEG
-decoder (csrc/jpeg_decode.cpp, plain C++).
+Sources: the reconstruction kernels and their C ABI (csrc/artefact_hip.hip), the warm-start
+CNN kernels (csrc/cnn.hip) and the host-side JPEG decoder (csrc/jpeg_decode.cpp, plain C++).
 
 -ffp-contract=off keeps every f32 multiply and add separately rounded, as the
 reference's default build does (backend/artefact-lib/utils/macros.rs:8-13);
@@ -14,7 +14,8 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = [os.path.join(HERE, "csrc", "artefact_hip.hip"), os.path.join(HERE, "csrc", "jpeg_decode.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", "artefact_hip.hip"), os.path.join(HERE, "csrc", "cnn.hip"),
+           os.path.join(HERE, "csrc", "jpeg_decode.cpp")]
 OUT = os.path.join(HERE, "libartefact_hip.so")
 
 HIPCC_FLAGS = [

@@ -59,6 +59,7 @@
 #include <vector>
 
 #include "../../include/artefact_hip.h"
+#include "cnn_internal.h"
 
 namespace {
 
@@ -1511,6 +1512,7 @@ struct artefact_hip_plan {
     int layout = 0;  // LAYOUT_*: which k_project variant runs
     int layout_fused = 0;  // the fused layout this plan's channels allow (LAYOUT_GENERIC if none)
     uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
+    float qmean[3] = {0, 0, 0};                   // mean quantisation step per channel (warm start)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
     ChanDev ch[3];
@@ -1940,11 +1942,14 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     size_t moff = 0;
     for (uint32_t c = 0; c < pl->C; c++) {
         const artefact_channel &k = chs[c];
+        double qsum = 0.0;
         for (int j = 0; j < 64; j++) {
             const float q = (float)k.quant[j];
             quant[c * 64 + j] = q;
             rqq[c * 64 + j] = 1.0 / (double)(q * q);  // q*q rounded to f32 first, as compute_step_prob.rs:29
+            qsum += q;
         }
+        pl->qmean[c] = (float)(qsum / 64.0);
         ok = ok && hipMemcpyAsync(pl->coef + moff, k.coefs, pl->M[c] * sizeof(int16_t), hipMemcpyHostToDevice,
                                   pl->stream) == hipSuccess;
         ChanDev &d = pl->ch[c];
@@ -2295,6 +2300,33 @@ int artefact_hip_plan_set_precision(artefact_hip_plan *pl, uint32_t precision)
     }
     pl->precision = precision;
     return ARTEFACT_OK;
+}
+
+int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
+{
+    if (!pl || !net) return ARTEFACT_E_ARGUMENT;
+    DeviceGuard guard(pl->device);
+    launch_init(pl, pl->stream);
+    const float *planes[3];
+    float *pred[3];
+    for (uint32_t c = 0; c < pl->C; c++) {
+        planes[c] = pl->buf[0] + (size_t)c * pl->N;
+        pred[c] = pl->ybuf + (size_t)c * pl->N;
+    }
+    int rc = cnn_predict_planes(net, planes, pl->qmean, pl->C, pl->H, pl->W, pred, pl->stream);
+    if (rc) return rc;
+    // Projection of the prediction: k_project with zero norms (no descent step) reads y =
+    // prediction and writes the projected image into buf[1] and the prob plane of its clamped
+    // coefficients, the state the loop carries into its next gradient.
+    bool ok = hipMemsetAsync(pl->norms, 0, 3 * sizeof(float), pl->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(pl->rnorm, 0, 3 * sizeof(double), pl->stream) == hipSuccess;
+    launch_project(pl, make_args(pl, 0, 0.0f, false), pl->stream);
+    ok = ok && hipMemcpyAsync(pl->buf[0], pl->buf[1], (size_t)pl->C * pl->N * sizeof(float),
+                              hipMemcpyDeviceToDevice, pl->stream) == hipSuccess;
+    ok = ok && hipGetLastError() == hipSuccess;
+    pl->cur = 0;
+    pl->initialized = ok;
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
 
 int artefact_hip_plan_tune(artefact_hip_plan *pl, artefact_tuning *out)

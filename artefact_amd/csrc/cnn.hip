@@ -1,0 +1,631 @@
+// Learned warm start: a small residual CNN, trained and run on the GPU with hand-written
+// kernels.  No counterpart in the reference (it is a start image for the reference's loop).
+//
+// Network: five 3x3 convolutions with zero padding, widths 2 -> 32 -> 32 -> 32 -> 32 -> 1,
+// ReLU after the first four, the output added to input feature 0.  Features of a plane:
+// f0 = x0 / 64 (the decoded, box-projected plane) and f1 = qstep / 64 (the channel's mean
+// quantisation step); the prediction is 64 * (f0 + net(f)).  Activations are NHWC bf16 with
+// the channel dimension padded to 16 or 32; weights are kept in f32 (Adam's master copy) and
+// repacked to bf16 operand layouts after every update.
+//
+// Convolution = implicit GEMM on v_mfma_f32_16x16x16_bf16 (A = weights [cout x cin] per tap,
+// B = input [cin x 16 pixels], C = output [cout x 16 pixels], f32 accumulation).  A wave owns
+// 16 consecutive pixels of a row and all output channels; its weights for all 9 taps sit in
+// registers.  The backward pass over the data is the same kernel on weights transposed
+// (cin <-> cout) and flipped (tap (dy, dx) -> (2 - dy, 2 - dx)).  Weight gradients: the same
+// MFMA with the pixels as the contraction dimension, per-workgroup partials reduced in a
+// fixed order.  Operand lane layouts (16x16x16, 64 lanes): A[m][k] at lane m + 16*(k/4) item
+// k%4; B[k][n] at lane n + 16*(k/4) item k%4; C[m][n] at lane n + 16*(m/4) item m%4.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/artefact_hip.h"
+#include "cnn_internal.h"
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int NLAYER = 5;
+constexpr int WIDTH[NLAYER + 1] = {2, 32, 32, 32, 32, 1};
+constexpr int CINP[NLAYER] = {16, 32, 32, 32, 32};   // padded input channels
+constexpr int COUTP[NLAYER] = {32, 32, 32, 32, 16};  // padded output channels
+constexpr float FEAT = 1.0f / 64.0f;
+constexpr int CT = 256;  // threads per workgroup (4 waves)
+
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Offsets of layer l's weights and biases in the flat parameter vector (w[cout][cin][3][3]
+// then b[cout], layer after layer).
+struct ParamLayout {
+    int w[NLAYER], b[NLAYER], total;
+};
+constexpr ParamLayout param_layout()
+{
+    ParamLayout p{};
+    int o = 0;
+    for (int l = 0; l < NLAYER; l++) {
+        p.w[l] = o;
+        o += WIDTH[l + 1] * WIDTH[l] * 9;
+        p.b[l] = o;
+        o += WIDTH[l + 1];
+    }
+    p.total = o;
+    return p;
+}
+constexpr ParamLayout PL = param_layout();
+
+// MODE_RELU: + bias, ReLU, bf16 out.  MODE_LINEAR: no bias, bf16 out (backward data).
+// MODE_FINAL: + bias, channel 0 only, f32 out = residual feature + conv (the prediction).
+enum { MODE_RELU = 0, MODE_LINEAR = 1, MODE_FINAL = 2 };
+
+template <int CI, int CO, int MODE>
+__global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, const bf16x4 *__restrict__ wpk,
+                                             const float *__restrict__ bias, uint16_t *__restrict__ out,
+                                             float *__restrict__ pred, const float *__restrict__ resid, int H,
+                                             int W)
+{
+    constexpr int KC = CI / 16, MH = CO / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b = blockIdx.z, y = blockIdx.y;
+    const int x0 = blockIdx.x * 64 + wave * 16;
+    if (x0 >= W) return;
+    bf16x4 wr[9][KC][MH];
+#pragma unroll
+    for (int t = 0; t < 9; t++)
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+            for (int mh = 0; mh < MH; mh++) wr[t][kc][mh] = wpk[((t * KC + kc) * MH + mh) * 64 + lane];
+    f32x4 acc[MH];
+#pragma unroll
+    for (int mh = 0; mh < MH; mh++) acc[mh] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const int n = lane & 15, kq = (lane >> 4) * 4;
+#pragma unroll
+    for (int dy = 0; dy < 3; dy++) {
+        const int yy = y + dy - 1;
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++) {
+            const int px = x0 + n + dx - 1;
+            const bool ok = yy >= 0 && yy < H && px >= 0 && px < W;
+            const size_t base = (((size_t)b * H + (ok ? yy : 0)) * W + (ok ? px : 0)) * CI + kq;
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) {
+                bf16x4 bv = {0, 0, 0, 0};
+                if (ok) bv = *reinterpret_cast<const bf16x4 *>(in + base + kc * 16);
+#pragma unroll
+                for (int mh = 0; mh < MH; mh++)
+                    acc[mh] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[dy * 3 + dx][kc][mh], bv, acc[mh], 0, 0, 0);
+            }
+        }
+    }
+    const int px = x0 + n;
+    if (px >= W) return;
+    const size_t p = ((size_t)b * H + y) * W + px;
+    if (MODE == MODE_FINAL) {
+        if (kq == 0) pred[p] = resid[p] + (acc[0][0] + bias[0]);
+        return;
+    }
+#pragma unroll
+    for (int mh = 0; mh < MH; mh++) {
+        uint16_t v[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            float r = acc[mh][t];
+            if (MODE == MODE_RELU) {
+                const int co = mh * 16 + kq + t;
+                r += bias[co];
+                r = r > 0.0f ? r : 0.0f;
+            }
+            v[t] = f2bf(r);
+        }
+        uint2 packed;
+        packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+        packed.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+        *reinterpret_cast<uint2 *>(out + p * CO + mh * 16 + kq) = packed;
+    }
+}
+
+// Weight and bias gradients of one layer over a range of 16-pixel row segments.  Wave w
+// handles taps w, w+4, w+8 of the ten "taps" (0..8 the 3x3 taps, 9 the bias, whose B operand
+// is 1 in column 0).  part: [workgroup][10][CO][CI] f32.
+template <int CI, int CO>
+__global__ __launch_bounds__(CT) void k_wgrad(const uint16_t *__restrict__ act, const uint16_t *__restrict__ gout,
+                                              float *__restrict__ part, int B, int H, int W)
+{
+    constexpr int KC = CI / 16, MH = CO / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = lane & 15, kq = (lane >> 4) * 4;
+    const long segs = (long)B * H * (W / 16);
+    const long s0 = segs * blockIdx.x / gridDim.x, s1 = segs * (blockIdx.x + 1) / gridDim.x;
+    f32x4 acc[3][MH][KC];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int mh = 0; mh < MH; mh++)
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) acc[i][mh][kc] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (long s = s0; s < s1; s++) {
+        const int xs = (int)(s % (W / 16)) * 16;
+        const long row = s / (W / 16);
+        const int y = (int)(row % H), b = (int)(row / H);
+        // A[m = co][k = pixel]: gout of pixels xs + kq + t.
+        bf16x4 av[MH];
+#pragma unroll
+        for (int mh = 0; mh < MH; mh++)
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                av[mh][t] = (short)gout[(((size_t)b * H + y) * W + xs + kq + t) * CO + mh * 16 + n];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int tap = wave + 4 * i;
+            if (tap > 9) break;
+            bf16x4 bv[KC];
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    short v = 0;
+                    if (tap == 9) {
+                        v = (kc == 0 && n == 0) ? (short)0x3F80 : (short)0;  // bf16 1.0 in column 0
+                    } else {
+                        const int yy = y + tap / 3 - 1, px = xs + kq + t + tap % 3 - 1;
+                        if (yy >= 0 && yy < H && px >= 0 && px < W)
+                            v = (short)act[(((size_t)b * H + yy) * W + px) * CI + kc * 16 + n];
+                    }
+                    bv[kc][t] = v;
+                }
+            }
+#pragma unroll
+            for (int mh = 0; mh < MH; mh++)
+#pragma unroll
+                for (int kc = 0; kc < KC; kc++)
+                    acc[i][mh][kc] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[mh], bv[kc], acc[i][mh][kc], 0, 0, 0);
+        }
+    }
+    // C[m = co][n = ci] at lane: co = mh*16 + kq + t, ci = kc*16 + n.
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const int tap = wave + 4 * i;
+        if (tap > 9) break;
+        float *dst = part + ((size_t)blockIdx.x * 10 + tap) * CO * CI;
+#pragma unroll
+        for (int mh = 0; mh < MH; mh++)
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) dst[(size_t)(mh * 16 + kq + t) * CI + kc * 16 + n] = acc[i][mh][kc][t];
+    }
+}
+
+// Sums the workgroup partials of layer l in a fixed order into the flat gradient vector.
+__global__ __launch_bounds__(CT) void k_wgrad_reduce(const float *part, int groups, int CI, int CO, int cin,
+                                                     int cout, float *gw, float *gb)
+{
+    const int i = blockIdx.x * CT + threadIdx.x;  // over cout * cin * 9 + cout
+    const int nw = cout * cin * 9;
+    if (i >= nw + cout) return;
+    int tap, co, ci;
+    if (i < nw) {
+        co = i / (cin * 9);
+        ci = (i / 9) % cin;
+        tap = i % 9;
+    } else {
+        co = i - nw;
+        ci = 0;
+        tap = 9;
+    }
+    double s = 0.0;
+    for (int g = 0; g < groups; g++) s += part[(((size_t)g * 10 + tap) * CO + co) * CI + ci];
+    if (i < nw) gw[i] = (float)s;
+    else gb[co] = (float)s;
+}
+
+// Weight packing into the A-operand layout [tap][kc][mh][lane] (4 bf16 per lane).  With
+// transpose set, packs the backward-data operator: A[m = ci][k = co] at tap (2-dy, 2-dx).
+__global__ __launch_bounds__(CT) void k_pack_weights(const float *w, int cin, int cout, int CI, int CO, int transpose,
+                                                     bf16x4 *dst)
+{
+    const int KCin = (transpose ? CO : CI) / 16, MHo = (transpose ? CI : CO) / 16;
+    const int total = 9 * KCin * MHo * 64;
+    const int i = blockIdx.x * CT + threadIdx.x;
+    if (i >= total) return;
+    const int lane = i & 63, mh = (i >> 6) % MHo, kc = (i >> 6) / MHo % KCin, tap = (i >> 6) / MHo / KCin;
+    bf16x4 v;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int m = mh * 16 + (lane & 15), k = kc * 16 + (lane >> 4) * 4 + t;
+        float x = 0.0f;
+        if (!transpose) {
+            if (m < cout && k < cin) x = w[((size_t)m * cin + k) * 9 + tap];
+        } else {
+            const int ftap = 8 - tap;  // (dy, dx) -> (2 - dy, 2 - dx)
+            if (k < cout && m < cin) x = w[((size_t)k * cin + m) * 9 + ftap];
+        }
+        v[t] = (short)f2bf(x);
+    }
+    dst[i] = v;
+}
+
+// Features of B planes: channel 0 = x0 / 64, channel 1 = qstep / 64, channels 2..15 = 0.
+__global__ __launch_bounds__(CT) void k_pack_input(const float *x0, const float *qstep, size_t plane, int B,
+                                                   uint16_t *a0, float *resid)
+{
+    const size_t i = (size_t)blockIdx.x * CT + threadIdx.x;
+    if (i >= plane * B) return;
+    const int b = (int)(i / plane);
+    const float f0 = x0[i] * FEAT;
+    resid[i] = f0;
+    uint16_t *o = a0 + i * 16;
+    o[0] = f2bf(f0);
+    o[1] = f2bf(qstep[b] * FEAT);
+#pragma unroll
+    for (int c = 2; c < 16; c++) o[c] = 0;
+}
+
+// Prediction back to image units.
+__global__ __launch_bounds__(CT) void k_unpack_pred(const float *pred, size_t n, float *out)
+{
+    const size_t i = (size_t)blockIdx.x * CT + threadIdx.x;
+    if (i < n) out[i] = pred[i] * 64.0f;
+}
+
+// MSE loss gradient in feature units: g = 2 (pred - target / 64) / n into channel 0 of a
+// 16-channel bf16 gradient; per-workgroup partial sums of the squared error (f64).
+__global__ __launch_bounds__(CT) void k_loss(const float *pred, const float *target, size_t n, uint16_t *g,
+                                             double *partial)
+{
+    __shared__ double red[CT];
+    const size_t i = (size_t)blockIdx.x * CT + threadIdx.x;
+    double e2 = 0.0;
+    if (i < n) {
+        const float d = pred[i] - target[i] * FEAT;
+        e2 = (double)d * d;
+        uint16_t *o = g + i * 16;
+        o[0] = f2bf(2.0f * d / (float)n);
+#pragma unroll
+        for (int c = 1; c < 16; c++) o[c] = 0;
+    }
+    red[threadIdx.x] = e2;
+    __syncthreads();
+    for (int s = CT / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// g *= (act > 0): the ReLU's backward pass, in place.
+__global__ __launch_bounds__(CT) void k_relu_bwd(uint16_t *g, const uint16_t *act, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * CT + threadIdx.x;
+    if (i < n && !(bf2f(act[i]) > 0.0f)) g[i] = 0;
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) on the f32 master parameters.
+__global__ __launch_bounds__(CT) void k_adam(float *p, const float *g, float *m, float *v, int n, float lr, float b1,
+                                             float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * CT + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+unsigned blocks(size_t n) { return (unsigned)((n + CT - 1) / CT); }
+
+}  // namespace
+
+struct artefact_cnn {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;  // PL.total each
+    bf16x4 *wf[NLAYER] = {}, *wb[NLAYER] = {};  // forward / backward-data operands
+    int adam_t = 0;
+    // workspace for a batch of B planes of H x W
+    size_t cap_px = 0;
+    uint16_t *act[NLAYER] = {};  // act[0] = features (16 ch), act[l] = output of layer l-1 (32 ch)
+    uint16_t *g32a = nullptr, *g32b = nullptr, *g16 = nullptr;
+    float *resid = nullptr, *pred = nullptr, *x0 = nullptr, *target = nullptr, *qstep = nullptr;
+    float *part = nullptr;
+    double *lpart = nullptr;
+    int wgroups = 0;
+    // resident training set
+    float *ds_x0 = nullptr, *ds_target = nullptr, *ds_q = nullptr;
+    uint32_t ds_count = 0, ds_h = 0, ds_w = 0;
+};
+
+namespace {
+
+bool ensure_workspace(artefact_cnn *net, size_t px)
+{
+    if (px <= net->cap_px) return true;
+    for (int l = 0; l < NLAYER; l++)
+        if (net->act[l]) (void)hipFree(net->act[l]);
+    void *bufs[] = {net->g32a, net->g32b, net->g16, net->resid, net->pred, net->x0, net->target, net->lpart};
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    bool ok = true;
+    ok = ok && hipMalloc(&net->act[0], px * 16 * 2) == hipSuccess;
+    for (int l = 1; l < NLAYER; l++) ok = ok && hipMalloc(&net->act[l], px * 32 * 2) == hipSuccess;
+    ok = ok && hipMalloc(&net->g32a, px * 32 * 2) == hipSuccess;
+    ok = ok && hipMalloc(&net->g32b, px * 32 * 2) == hipSuccess;
+    ok = ok && hipMalloc(&net->g16, px * 16 * 2) == hipSuccess;
+    ok = ok && hipMalloc(&net->resid, px * 4) == hipSuccess;
+    ok = ok && hipMalloc(&net->pred, px * 4) == hipSuccess;
+    ok = ok && hipMalloc(&net->x0, px * 4) == hipSuccess;
+    ok = ok && hipMalloc(&net->target, px * 4) == hipSuccess;
+    ok = ok && hipMalloc(&net->lpart, blocks(px) * sizeof(double)) == hipSuccess;
+    net->cap_px = ok ? px : 0;
+    return ok;
+}
+
+void repack(artefact_cnn *net)
+{
+    for (int l = 0; l < NLAYER; l++) {
+        const int nf = 9 * (CINP[l] / 16) * (COUTP[l] / 16) * 64;
+        hipLaunchKernelGGL(k_pack_weights, dim3(blocks(nf)), dim3(CT), 0, net->stream, net->params + PL.w[l],
+                           WIDTH[l], WIDTH[l + 1], CINP[l], COUTP[l], 0, net->wf[l]);
+        hipLaunchKernelGGL(k_pack_weights, dim3(blocks(nf)), dim3(CT), 0, net->stream, net->params + PL.w[l],
+                           WIDTH[l], WIDTH[l + 1], CINP[l], COUTP[l], 1, net->wb[l]);
+    }
+}
+
+template <int CI, int CO, int MODE>
+void conv(artefact_cnn *net, const uint16_t *in, const bf16x4 *w, const float *bias, uint16_t *out, float *pred,
+          const float *resid, int B, int H, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_conv<CI, CO, MODE>), dim3((W + 63) / 64, H, B), dim3(CT), 0, s, in, w, bias, out, pred,
+                       resid, H, W);
+}
+
+// Forward pass on the features in act[0] / resid; the prediction (feature units) in pred.
+void forward(artefact_cnn *net, int B, int H, int W, hipStream_t s)
+{
+    conv<16, 32, MODE_RELU>(net, net->act[0], net->wf[0], net->params + PL.b[0], net->act[1], nullptr, nullptr, B, H,
+                            W, s);
+    conv<32, 32, MODE_RELU>(net, net->act[1], net->wf[1], net->params + PL.b[1], net->act[2], nullptr, nullptr, B, H,
+                            W, s);
+    conv<32, 32, MODE_RELU>(net, net->act[2], net->wf[2], net->params + PL.b[2], net->act[3], nullptr, nullptr, B, H,
+                            W, s);
+    conv<32, 32, MODE_RELU>(net, net->act[3], net->wf[3], net->params + PL.b[3], net->act[4], nullptr, nullptr, B, H,
+                            W, s);
+    conv<32, 16, MODE_FINAL>(net, net->act[4], net->wf[4], net->params + PL.b[4], nullptr, net->pred, net->resid, B,
+                             H, W, s);
+}
+
+template <int CI, int CO>
+void wgrad(artefact_cnn *net, int l, const uint16_t *act, const uint16_t *gout, int B, int H, int W)
+{
+    hipLaunchKernelGGL((k_wgrad<CI, CO>), dim3(net->wgroups), dim3(CT), 0, net->stream, act, gout, net->part, B, H, W);
+    const int n = WIDTH[l + 1] * WIDTH[l] * 9 + WIDTH[l + 1];
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks(n)), dim3(CT), 0, net->stream, (const float *)net->part,
+                       net->wgroups, CI, CO, WIDTH[l], WIDTH[l + 1], net->grads + PL.w[l], net->grads + PL.b[l]);
+}
+
+// Loss and gradients for the batch whose features, residual and target are on the device.
+bool backward(artefact_cnn *net, int B, int H, int W, double *loss)
+{
+    const size_t px = (size_t)B * H * W;
+    const unsigned nb = blocks(px);
+    hipLaunchKernelGGL(k_loss, dim3(nb), dim3(CT), 0, net->stream, (const float *)net->pred,
+                       (const float *)net->target, px, net->g16, net->lpart);
+    // layer 4 (32 -> 16 padded): weights from act[4]; data gradient into g32a
+    wgrad<32, 16>(net, 4, net->act[4], net->g16, B, H, W);
+    conv<16, 32, MODE_LINEAR>(net, net->g16, net->wb[4], nullptr, net->g32a, nullptr, nullptr, B, H, W,
+                              net->stream);
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks(px * 32)), dim3(CT), 0, net->stream, net->g32a,
+                       (const uint16_t *)net->act[4], px * 32);
+    uint16_t *g = net->g32a, *gn = net->g32b;
+    for (int l = 3; l >= 1; l--) {
+        wgrad<32, 32>(net, l, net->act[l], g, B, H, W);
+        conv<32, 32, MODE_LINEAR>(net, g, net->wb[l], nullptr, gn, nullptr, nullptr, B, H, W, net->stream);
+        hipLaunchKernelGGL(k_relu_bwd, dim3(blocks(px * 32)), dim3(CT), 0, net->stream, gn,
+                           (const uint16_t *)net->act[l], px * 32);
+        std::swap(g, gn);
+    }
+    wgrad<16, 32>(net, 0, net->act[0], g, B, H, W);
+    std::vector<double> lp(nb);
+    bool ok = hipMemcpyAsync(lp.data(), net->lpart, nb * sizeof(double), hipMemcpyDeviceToHost, net->stream) ==
+              hipSuccess;
+    ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+    double s = 0.0;
+    for (double v : lp) s += v;
+    if (loss) *loss = s / (double)px;
+    return ok;
+}
+
+bool upload_batch(artefact_cnn *net, const float *x0, const float *q, const float *target, int B, int H, int W,
+                  bool device)
+{
+    const size_t px = (size_t)B * H * W;
+    if (!ensure_workspace(net, px)) return false;
+    const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    bool ok = hipMemcpyAsync(net->x0, x0, px * 4, kind, net->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(net->qstep, q, (size_t)B * 4, kind, net->stream) == hipSuccess;
+    if (target) ok = ok && hipMemcpyAsync(net->target, target, px * 4, kind, net->stream) == hipSuccess;
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks(px)), dim3(CT), 0, net->stream, (const float *)net->x0,
+                       (const float *)net->qstep, (size_t)H * W, B, net->act[0], net->resid);
+    return ok && hipGetLastError() == hipSuccess;
+}
+
+}  // namespace
+
+int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const float *qsteps, uint32_t nplanes,
+                       uint32_t h, uint32_t w, float *const *out, hipStream_t s)
+{
+    if (!net || !planes || !qsteps || !out || !nplanes || nplanes > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
+    const size_t plane = (size_t)h * w;
+    if (!ensure_workspace(net, plane)) return ARTEFACT_E_HIP;
+    bool ok = true;
+    for (uint32_t i = 0; i < nplanes && ok; i++) {
+        ok = hipMemcpyAsync(net->qstep, qsteps + i, 4, hipMemcpyHostToDevice, s) == hipSuccess;
+        hipLaunchKernelGGL(k_pack_input, dim3(blocks(plane)), dim3(CT), 0, s, planes[i], (const float *)net->qstep,
+                           plane, 1, net->act[0], net->resid);
+        forward(net, 1, (int)h, (int)w, s);
+        hipLaunchKernelGGL(k_unpack_pred, dim3(blocks(plane)), dim3(CT), 0, s, (const float *)net->pred, plane,
+                           out[i]);
+        ok = ok && hipGetLastError() == hipSuccess;
+    }
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+extern "C" {
+
+uint32_t artefact_cnn_param_count(void) { return (uint32_t)PL.total; }
+
+int artefact_cnn_create(const float *params, artefact_cnn **out)
+{
+    if (!out || !params) return ARTEFACT_E_ARGUMENT;
+    *out = nullptr;
+    artefact_cnn *net = new (std::nothrow) artefact_cnn;
+    if (!net) return ARTEFACT_E_HIP;
+    bool ok = hipGetDevice(&net->device) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) == hipSuccess;
+    const size_t pb = (size_t)PL.total * 4;
+    ok = ok && hipMalloc(&net->params, pb) == hipSuccess && hipMalloc(&net->grads, pb) == hipSuccess;
+    ok = ok && hipMalloc(&net->m, pb) == hipSuccess && hipMalloc(&net->v, pb) == hipSuccess;
+    ok = ok && hipMalloc(&net->qstep, 64 * 4) == hipSuccess;
+    for (int l = 0; l < NLAYER && ok; l++) {
+        const size_t nf = (size_t)9 * (CINP[l] / 16) * (COUTP[l] / 16) * 64 * sizeof(bf16x4);
+        ok = hipMalloc(&net->wf[l], nf) == hipSuccess && hipMalloc(&net->wb[l], nf) == hipSuccess;
+    }
+    net->wgroups = 512;
+    ok = ok && hipMalloc(&net->part, (size_t)net->wgroups * 10 * 32 * 32 * 4) == hipSuccess;
+    ok = ok && hipMemcpyAsync(net->params, params, pb, hipMemcpyHostToDevice, net->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(net->m, 0, pb, net->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(net->v, 0, pb, net->stream) == hipSuccess;
+    if (ok) repack(net);
+    ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+    if (!ok) {
+        artefact_cnn_destroy(net);
+        return ARTEFACT_E_HIP;
+    }
+    *out = net;
+    return ARTEFACT_OK;
+}
+
+int artefact_cnn_get_params(artefact_cnn *net, float *out)
+{
+    if (!net || !out) return ARTEFACT_E_ARGUMENT;
+    bool ok = hipMemcpyAsync(out, net->params, (size_t)PL.total * 4, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *qsteps, uint32_t batch, uint32_t h,
+                         uint32_t w, float *pred)
+{
+    if (!net || !x0 || !qsteps || !pred || !batch || batch > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
+    if (!upload_batch(net, x0, qsteps, nullptr, (int)batch, (int)h, (int)w, false)) return ARTEFACT_E_HIP;
+    const size_t px = (size_t)batch * h * w;
+    forward(net, (int)batch, (int)h, (int)w, net->stream);
+    hipLaunchKernelGGL(k_unpack_pred, dim3(blocks(px)), dim3(CT), 0, net->stream, (const float *)net->pred, px,
+                       net->x0);
+    bool ok = hipGetLastError() == hipSuccess;
+    ok = ok && hipMemcpyAsync(pred, net->x0, px * 4, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+int artefact_cnn_gradients(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+                           uint32_t batch, uint32_t h, uint32_t w, double *loss, float *grads)
+{
+    if (!net || !x0 || !qsteps || !target || !batch || batch > 64 || !h || !w || w % 16) return ARTEFACT_E_ARGUMENT;
+    if (!upload_batch(net, x0, qsteps, target, (int)batch, (int)h, (int)w, false)) return ARTEFACT_E_HIP;
+    forward(net, (int)batch, (int)h, (int)w, net->stream);
+    if (!backward(net, (int)batch, (int)h, (int)w, loss)) return ARTEFACT_E_HIP;
+    if (grads) {
+        bool ok = hipMemcpyAsync(grads, net->grads, (size_t)PL.total * 4, hipMemcpyDeviceToHost, net->stream) ==
+                  hipSuccess;
+        ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+        if (!ok) return ARTEFACT_E_HIP;
+    }
+    return ARTEFACT_OK;
+}
+
+int artefact_cnn_set_dataset(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+                             uint32_t count, uint32_t h, uint32_t w)
+{
+    if (!net || !x0 || !qsteps || !target || !count || !h || !w || w % 16) return ARTEFACT_E_ARGUMENT;
+    for (float *p : {net->ds_x0, net->ds_target, net->ds_q})
+        if (p) (void)hipFree(p);
+    net->ds_x0 = net->ds_target = net->ds_q = nullptr;
+    const size_t px = (size_t)count * h * w;
+    bool ok = hipMalloc(&net->ds_x0, px * 4) == hipSuccess && hipMalloc(&net->ds_target, px * 4) == hipSuccess &&
+              hipMalloc(&net->ds_q, (size_t)count * 4) == hipSuccess;
+    ok = ok && hipMemcpyAsync(net->ds_x0, x0, px * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(net->ds_target, target, px * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(net->ds_q, qsteps, (size_t)count * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+    net->ds_count = ok ? count : 0;
+    net->ds_h = h;
+    net->ds_w = w;
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+int artefact_cnn_train_step(artefact_cnn *net, const uint32_t *indices, uint32_t batch, float lr, double *loss)
+{
+    if (!net || !indices || !batch || batch > 64 || !net->ds_count) return ARTEFACT_E_ARGUMENT;
+    const int H = (int)net->ds_h, W = (int)net->ds_w;
+    const size_t plane = (size_t)H * W;
+    if (!ensure_workspace(net, plane * batch)) return ARTEFACT_E_HIP;
+    bool ok = true;
+    std::vector<float> q(batch);
+    for (uint32_t i = 0; i < batch && ok; i++) {
+        if (indices[i] >= net->ds_count) return ARTEFACT_E_ARGUMENT;
+        ok = hipMemcpyAsync(net->x0 + i * plane, net->ds_x0 + indices[i] * plane, plane * 4,
+                            hipMemcpyDeviceToDevice, net->stream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(net->target + i * plane, net->ds_target + indices[i] * plane, plane * 4,
+                                  hipMemcpyDeviceToDevice, net->stream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(net->qstep + i, net->ds_q + indices[i], 4, hipMemcpyDeviceToDevice, net->stream) ==
+                       hipSuccess;
+    }
+    hipLaunchKernelGGL(k_pack_input, dim3(blocks(plane * batch)), dim3(CT), 0, net->stream, (const float *)net->x0,
+                       (const float *)net->qstep, plane, (int)batch, net->act[0], net->resid);
+    forward(net, (int)batch, H, W, net->stream);
+    ok = ok && backward(net, (int)batch, H, W, loss);
+    net->adam_t++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - powf(b1, (float)net->adam_t), c2 = 1.0f - powf(b2, (float)net->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3(blocks(PL.total)), dim3(CT), 0, net->stream, net->params,
+                       (const float *)net->grads, net->m, net->v, PL.total, lr, b1, b2, 1e-8f, c1, c2);
+    repack(net);
+    ok = ok && hipGetLastError() == hipSuccess;
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
+}
+
+void artefact_cnn_destroy(artefact_cnn *net)
+{
+    if (!net) return;
+    if (net->stream) (void)hipStreamSynchronize(net->stream);
+    std::vector<void *> bufs{net->params, net->grads, net->m, net->v, net->qstep, net->part, net->g32a, net->g32b,
+                             net->g16, net->resid, net->pred, net->x0, net->target, net->lpart, net->ds_x0,
+                             net->ds_target, net->ds_q};
+    for (int l = 0; l < NLAYER; l++) {
+        bufs.push_back(net->wf[l]);
+        bufs.push_back(net->wb[l]);
+        bufs.push_back(net->act[l]);
+    }
+    for (void *p : bufs)
+        if (p) (void)hipFree(p);
+    if (net->stream) (void)hipStreamDestroy(net->stream);
+    delete net;
+}
+
+}  // extern "C"

@@ -1,0 +1,207 @@
+"""Learned warm start: a residual CNN whose prediction starts the reconstruction loop.
+
+The network (csrc/cnn.hip) maps a decoded plane and its channel's mean quantisation step to
+a prediction of the converged plane.  Training pairs come from the solver itself: the
+decoded start image of a synthetic crop (already inside the quantisation box) and the plain
+TGV loop's output after ``target_iterations`` iterations.  Forward, backward and Adam run on
+the GPU.  ``warm_solve`` starts a plan from the projected prediction, runs its iterations and
+falls back to the plain loop when the warm result's objective is higher than the plain
+result's (the plain solver is the judge).  No reference counterpart.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib, synth
+from .batch import Batch, BatchImage
+from .compute import Plan
+
+_FP = ctypes.POINTER(ctypes.c_float)
+WIDTHS = (2, 32, 32, 32, 32, 1)
+WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-initialised flat parameters (per layer w[cout][cin][3][3], then b[cout]); the last
+    layer is scaled by 0.1 so that the untrained network is close to the identity map."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for li, (cin, cout) in enumerate(zip(WIDTHS[:-1], WIDTHS[1:])):
+        w = rng.normal(0.0, np.sqrt(2.0 / (cin * 9)), size=(cout, cin, 3, 3))
+        if li == len(WIDTHS) - 2:
+            w *= 0.1
+        out += [w.reshape(-1), np.zeros(cout)]
+    return np.concatenate(out).astype(np.float32)
+
+
+def split_params(flat: np.ndarray):
+    """Flat vector -> ([w_l (cout, cin, 3, 3)], [b_l]) in float64."""
+    ws, bs, o = [], [], 0
+    for cin, cout in zip(WIDTHS[:-1], WIDTHS[1:]):
+        n = cout * cin * 9
+        ws.append(flat[o:o + n].astype(np.float64).reshape(cout, cin, 3, 3))
+        o += n
+        bs.append(flat[o:o + cout].astype(np.float64))
+        o += cout
+    return ws, bs
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+class WarmStartNet:
+    def __init__(self, params: np.ndarray | None = None):
+        self._lib = _lib.load()
+        n = self._lib.artefact_cnn_param_count()
+        p = init_params() if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != n:
+            raise ValueError(f"expected {n} parameters, got {p.size}")
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.artefact_cnn_create(_fp(p), ctypes.byref(h)), "artefact_cnn_create")
+        self._h = h
+        self.n = n
+
+    @classmethod
+    def load(cls, path: str = WEIGHTS_FILE) -> "WarmStartNet":
+        with np.load(path, allow_pickle=False) as f:
+            return cls(f["params"])
+
+    def save(self, path: str, **meta) -> None:
+        np.savez(path, params=self.params(), **{k: np.asarray(v) for k, v in meta.items()})
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float32)
+        _lib.check(self._lib.artefact_cnn_get_params(self._h, _fp(out)), "artefact_cnn_get_params")
+        return out
+
+    def forward(self, x0: np.ndarray, qsteps) -> np.ndarray:
+        """Predictions (image units) for planes x0 (B, H, W)."""
+        x = np.ascontiguousarray(x0, dtype=np.float32)
+        q = np.ascontiguousarray(qsteps, dtype=np.float32)
+        out = np.empty_like(x)
+        B, H, W = x.shape
+        _lib.check(self._lib.artefact_cnn_forward(self._h, _fp(x), _fp(q), B, H, W, _fp(out)), "artefact_cnn_forward")
+        return out
+
+    def gradients(self, x0, qsteps, target):
+        """(loss, flat parameter gradients) of the mean squared error in units of 1/64."""
+        x = np.ascontiguousarray(x0, dtype=np.float32)
+        q = np.ascontiguousarray(qsteps, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.empty(self.n, dtype=np.float32)
+        loss = ctypes.c_double()
+        B, H, W = x.shape
+        _lib.check(self._lib.artefact_cnn_gradients(self._h, _fp(x), _fp(q), _fp(t), B, H, W, ctypes.byref(loss),
+                                                    _fp(g)), "artefact_cnn_gradients")
+        return loss.value, g
+
+    def set_dataset(self, x0, qsteps, target) -> None:
+        x = np.ascontiguousarray(x0, dtype=np.float32)
+        q = np.ascontiguousarray(qsteps, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        B, H, W = x.shape
+        _lib.check(self._lib.artefact_cnn_set_dataset(self._h, _fp(x), _fp(q), _fp(t), B, H, W),
+                   "artefact_cnn_set_dataset")
+        self.count = B
+
+    def train_step(self, indices, lr: float) -> float:
+        idx = np.ascontiguousarray(indices, dtype=np.uint32)
+        loss = ctypes.c_double()
+        _lib.check(self._lib.artefact_cnn_train_step(self._h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                     idx.size, float(lr), ctypes.byref(loss)),
+                   "artefact_cnn_train_step")
+        return loss.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.artefact_cnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def qstep(coef) -> float:
+    return float(np.asarray(coef.quant_table, dtype=np.float64).mean())
+
+
+def training_pairs(count: int, size: int = 256, seed: int = 0, target_iterations: int = 300,
+                   qualities=(20, 90), weight: float = 0.3, pweight: float = 0.001):
+    """``count`` synthetic 4:2:0 crops of size x size at qualities drawn uniformly in the
+    range; every channel plane (grid resolution) is one sample.  Returns (x0, q, target) with
+    x0 the decoded start plane and target the plain TGV loop's output, all float32."""
+    rng = np.random.default_rng(seed)
+    imgs, starts, qs = [], [], []
+    for i in range(count):
+        quality = int(rng.integers(qualities[0], qualities[1] + 1))
+        coefs = synth.encode(synth.synth_rgb(size, size, seed * 100003 + i), subsampling="420", quality=quality)
+        imgs.append(BatchImage(coefs, weight, [pweight] * 3, target_iterations, size, size, "tgv"))
+        with Plan(coefs, weight, [pweight] * 3, 0, size, size) as plan:
+            plan.init()
+            starts += plan.download()
+        qs += [qstep(c) for c in coefs]
+    with Batch(imgs, streams=4, tune=False) as b:
+        b.run()
+        targets = [p for i in range(count) for p in b.download(i)]
+    shape = (len(starts), size, size)
+    return (np.stack(starts).reshape(shape), np.asarray(qs, np.float32), np.stack(targets).reshape(shape))
+
+
+def train(net: WarmStartNet, x0, q, target, steps: int, batch: int = 16, lr: float = 1e-3, seed: int = 0,
+          log_every: int = 100, log=print):
+    """Adam over random mini-batches of the resident set; cosine-decayed learning rate."""
+    net.set_dataset(x0, q, target)
+    rng = np.random.default_rng(seed)
+    losses = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        idx = rng.integers(0, x0.shape[0], size=batch)
+        cur = lr * 0.5 * (1.0 + np.cos(np.pi * s / steps))
+        losses.append(net.train_step(idx, cur))
+        if log and (s % log_every == 0 or s == steps - 1):
+            log(f"step {s} loss {losses[-1]:.6g} ({time.perf_counter() - t0:.1f} s)")
+    return losses
+
+
+@dataclass
+class WarmResult:
+    planes: list
+    objective: float
+    used_fallback: bool
+    warm_objective: float
+
+
+def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective: float | None = None,
+               weight: float = 0.3, pweight=(0.001, 0.001, 0.001), plain_iterations: int = 100) -> WarmResult:
+    """CNN prediction -> projection -> ``iterations`` plain iterations; if the result's objective
+    is higher than the plain ``plain_iterations`` result's, the plain result is returned."""
+    with Plan(coefs, weight, list(pweight), iterations, W, H) as plan:
+        _lib.check(plan._lib.artefact_hip_plan_warm_init(plan._h, net._h), "artefact_hip_plan_warm_init")
+        plan.iterate()
+        planes = plan.download()
+        fw, _ = plan.objective()
+    if plain_objective is None:
+        with Plan(coefs, weight, list(pweight), plain_iterations, W, H) as plan:
+            plan.init()
+            plan.iterate()
+            plain = plan.download()
+            plain_objective, _ = plan.objective()
+    else:
+        plain = None
+    if fw <= plain_objective:
+        return WarmResult(planes, fw, False, fw)
+    if plain is None:
+        with Plan(coefs, weight, list(pweight), plain_iterations, W, H) as plan:
+            plan.init()
+            plan.iterate()
+            plain = plan.download()
+    return WarmResult(plain, plain_objective, True, fw)

@@ -198,6 +198,35 @@ typedef struct {
 int artefact_hip_process(const artefact_process_params *params, const artefact_channel *channels,
                          uint8_t *out_rgb);
 
+/* ---- Learned warm start (no reference counterpart) ----
+ * A residual CNN (five 3x3 convolutions, widths 2 -> 32 -> 32 -> 32 -> 32 -> 1, ReLU, output
+ * added to the decoded plane) that predicts the converged plane from the decoded one and the
+ * channel's mean quantisation step.  Parameters are a flat f32 vector: per layer w[cout][cin][3][3]
+ * then b[cout].  Forward, backward and Adam run as HIP kernels (BF16 MFMA, f32 accumulation). */
+typedef struct artefact_cnn artefact_cnn;
+uint32_t artefact_cnn_param_count(void);
+int artefact_cnn_create(const float *params, artefact_cnn **out_net);
+int artefact_cnn_get_params(artefact_cnn *net, float *out_params);
+/* Prediction (image units) for `batch` host planes of h x w with one quantisation step each. */
+int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *qsteps, uint32_t batch, uint32_t h,
+                         uint32_t w, float *out_pred);
+/* Mean squared error (in units of 1/64) and its parameter gradients for a host batch; w must
+ * be a multiple of 16.  grads may be NULL. */
+int artefact_cnn_gradients(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+                           uint32_t batch, uint32_t h, uint32_t w, double *loss, float *grads);
+/* Keeps a training set (count planes of h x w, targets, steps) resident in HBM. */
+int artefact_cnn_set_dataset(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+                             uint32_t count, uint32_t h, uint32_t w);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the samples `indices` of the set. */
+int artefact_cnn_train_step(artefact_cnn *net, const uint32_t *indices, uint32_t batch, float lr, double *loss);
+void artefact_cnn_destroy(artefact_cnn *net);
+
+/* Warm start of a plan: the decoded start image, the CNN's prediction for every channel
+ * plane, one projection into the quantisation box (compute_projection.rs:8-122), and that
+ * image as both x and x_prev.  artefact_hip_plan_iterate then runs the plan's iterations
+ * from there. */
+int artefact_hip_plan_warm_init(artefact_hip_plan *plan, artefact_cnn *net);
+
 /* ---- JPEG input: Jpeg::from (backend/artefact-lib/jpeg/zune.rs:5-62) on the host ----
  * Decodes baseline/extended sequential and progressive Huffman JPEGs (8-bit, 1-4 components,
  * sampling factors 1 or 2) to the reference's Coefficient layout: per component the plane

@@ -1,0 +1,86 @@
+"""Warm-start CNN kernels (csrc/cnn.hip) against the CPU reference (oracle/cnn.py).
+
+BF16 activations and weights with f32 accumulation are not bit-identical to the float64
+reference, so the checks are relative: the predicted residual within 3 % (relative L2), each
+layer's weight and bias gradients with cosine similarity >= 0.99 and relative L2 error
+<= 8 %, and Adam steps on a fixed batch lower the loss.
+"""
+import numpy as np
+import pytest
+
+from artefact_amd import warmstart
+from oracle import cnn
+
+pytestmark = pytest.mark.gpu
+
+
+def features(x0, q):
+    B, H, W = x0.shape
+    f = np.zeros((B, H, W, 2))
+    f[..., 0] = x0 / 64.0
+    f[..., 1] = (np.asarray(q)[:, None, None] / 64.0) * np.ones((B, H, W))
+    return f
+
+
+def params_with_bias(seed):
+    p = warmstart.init_params(seed).astype(np.float64)
+    ws, bs = warmstart.split_params(p)
+    rng = np.random.default_rng(seed + 7)
+    for b in bs:
+        b[:] = rng.normal(scale=0.05, size=b.shape)
+    flat = np.concatenate([np.concatenate([w.reshape(-1), b]) for w, b in zip(ws, bs)])
+    return flat.astype(np.float32), ws, bs
+
+
+def batch(seed, B=2, H=24, W=48):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:H, 0:W]
+    x0 = np.stack([60 * np.sin(xx / (5 + i)) + 40 * np.cos(yy / 7) + rng.normal(0, 8, (H, W)) for i in range(B)])
+    q = rng.uniform(5, 40, size=B)
+    t = x0 + rng.normal(0, 4, x0.shape)
+    return x0.astype(np.float32), q.astype(np.float32), t.astype(np.float32)
+
+
+def test_forward_matches_oracle():
+    flat, ws, bs = params_with_bias(1)
+    x0, q, _ = batch(2)
+    print("case cnn forward", flush=True)
+    net = warmstart.WarmStartNet(flat)
+    got = net.forward(x0, q)
+    net.close()
+    ref, _, _ = cnn.forward(features(x0, q), ws, bs)
+    ref = ref * 64.0
+    r_got, r_ref = got - x0, ref - x0
+    assert np.linalg.norm(r_got - r_ref) <= 0.03 * np.linalg.norm(r_ref)
+
+
+def test_gradients_match_oracle():
+    flat, ws, bs = params_with_bias(3)
+    x0, q, t = batch(4)
+    print("case cnn gradients", flush=True)
+    net = warmstart.WarmStartNet(flat)
+    loss, g = net.gradients(x0, q, t)
+    net.close()
+    rloss, gws, gbs = cnn.loss_and_grads(features(x0, q), t / 64.0, ws, bs)
+    assert abs(loss - rloss) <= 0.03 * rloss
+    gw, gb = warmstart.split_params(g)
+    for l in range(len(ws)):
+        for a, b in ((gw[l], gws[l]), (gb[l], gbs[l])):
+            a, b = a.reshape(-1), b.reshape(-1)
+            cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30))
+            rel = float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30))
+            print(f"layer {l}: cos {cos:.5f} rel {rel:.4f}")
+            assert cos >= 0.99 and rel <= 0.08, (l, cos, rel)
+
+
+def test_adam_lowers_the_loss():
+    """A learnable target (a pointwise nonlinearity of the input): 150 Adam steps on one batch
+    cut the loss at least fourfold."""
+    x0, q, _ = batch(5, B=4, H=32, W=32)
+    t = (x0 + 10.0 * np.tanh(x0 / 30.0)).astype(np.float32)
+    net = warmstart.WarmStartNet(warmstart.init_params(0))
+    net.set_dataset(x0, q, t)
+    losses = [net.train_step(np.arange(4), 2e-3) for _ in range(150)]
+    net.close()
+    print(f"loss {losses[0]:.5g} -> {losses[-1]:.5g}")
+    assert losses[-1] < 0.25 * losses[0], (losses[0], losses[-1])
