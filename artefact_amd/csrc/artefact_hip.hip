@@ -2302,6 +2302,18 @@ int artefact_hip_plan_set_precision(artefact_hip_plan *pl, uint32_t precision)
     return ARTEFACT_OK;
 }
 
+int artefact_hip_plan_set_step_iterations(artefact_hip_plan *pl, uint32_t n)
+{
+    if (!pl) return ARTEFACT_E_ARGUMENT;
+    const float radius = sqrtf((float)pl->N) / 2.0f;
+    const uint32_t k = n ? n : pl->iterations;
+    pl->step = radius / sqrtf(1.0f + (float)k);  // pipeline_scalar/mod.rs:45,64-74 with k in place of iterations
+    DeviceGuard guard(pl->device);
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    drop_graph(pl);
+    return ARTEFACT_OK;
+}
+
 int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
 {
     if (!pl || !net) return ARTEFACT_E_ARGUMENT;

@@ -70,6 +70,10 @@ constexpr ParamLayout PL = param_layout();
 // MODE_FINAL: + bias, channel 0 only, f32 out = residual feature + conv (the prediction).
 enum { MODE_RELU = 0, MODE_LINEAR = 1, MODE_FINAL = 2 };
 
+// Pixels of one row per workgroup: 4 waves x 16 pixels, NSEG times (the weights stay in
+// registers across the segments).
+constexpr int NSEG = 8;
+
 template <int CI, int CO, int MODE>
 __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, const bf16x4 *__restrict__ wpk,
                                              const float *__restrict__ bias, uint16_t *__restrict__ out,
@@ -79,8 +83,8 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
     constexpr int KC = CI / 16, MH = CO / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.z, y = blockIdx.y;
-    const int x0 = blockIdx.x * 64 + wave * 16;
-    if (x0 >= W) return;
+    const int xb = blockIdx.x * 64 * NSEG + wave * 16;
+    if (xb >= W) return;
     bf16x4 wr[9][KC][MH];
 #pragma unroll
     for (int t = 0; t < 9; t++)
@@ -88,52 +92,57 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
         for (int kc = 0; kc < KC; kc++)
 #pragma unroll
             for (int mh = 0; mh < MH; mh++) wr[t][kc][mh] = wpk[((t * KC + kc) * MH + mh) * 64 + lane];
-    f32x4 acc[MH];
-#pragma unroll
-    for (int mh = 0; mh < MH; mh++) acc[mh] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const int n = lane & 15, kq = (lane >> 4) * 4;
+    for (int seg = 0; seg < NSEG; seg++) {
+        const int x0 = xb + seg * 64;
+        if (x0 >= W) break;
+        f32x4 acc[MH];
 #pragma unroll
-    for (int dy = 0; dy < 3; dy++) {
-        const int yy = y + dy - 1;
+        for (int mh = 0; mh < MH; mh++) acc[mh] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int dx = 0; dx < 3; dx++) {
-            const int px = x0 + n + dx - 1;
-            const bool ok = yy >= 0 && yy < H && px >= 0 && px < W;
-            const size_t base = (((size_t)b * H + (ok ? yy : 0)) * W + (ok ? px : 0)) * CI + kq;
+        for (int dy = 0; dy < 3; dy++) {
+            const int yy = y + dy - 1;
 #pragma unroll
-            for (int kc = 0; kc < KC; kc++) {
-                bf16x4 bv = {0, 0, 0, 0};
-                if (ok) bv = *reinterpret_cast<const bf16x4 *>(in + base + kc * 16);
+            for (int dx = 0; dx < 3; dx++) {
+                const int px = x0 + n + dx - 1;
+                const bool ok = yy >= 0 && yy < H && px >= 0 && px < W;
+                const size_t base = (((size_t)b * H + (ok ? yy : 0)) * W + (ok ? px : 0)) * CI + kq;
 #pragma unroll
-                for (int mh = 0; mh < MH; mh++)
-                    acc[mh] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[dy * 3 + dx][kc][mh], bv, acc[mh], 0, 0, 0);
+                for (int kc = 0; kc < KC; kc++) {
+                    bf16x4 bv = {0, 0, 0, 0};
+                    if (ok) bv = *reinterpret_cast<const bf16x4 *>(in + base + kc * 16);
+#pragma unroll
+                    for (int mh = 0; mh < MH; mh++)
+                        acc[mh] =
+                            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[dy * 3 + dx][kc][mh], bv, acc[mh], 0, 0, 0);
+                }
             }
         }
-    }
-    const int px = x0 + n;
-    if (px >= W) return;
-    const size_t p = ((size_t)b * H + y) * W + px;
-    if (MODE == MODE_FINAL) {
-        if (kq == 0) pred[p] = resid[p] + (acc[0][0] + bias[0]);
-        return;
-    }
-#pragma unroll
-    for (int mh = 0; mh < MH; mh++) {
-        uint16_t v[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            float r = acc[mh][t];
-            if (MODE == MODE_RELU) {
-                const int co = mh * 16 + kq + t;
-                r += bias[co];
-                r = r > 0.0f ? r : 0.0f;
-            }
-            v[t] = f2bf(r);
+        const int px = x0 + n;
+        if (px >= W) continue;
+        const size_t p = ((size_t)b * H + y) * W + px;
+        if (MODE == MODE_FINAL) {
+            if (kq == 0) pred[p] = resid[p] + (acc[0][0] + bias[0]);
+            continue;
         }
-        uint2 packed;
-        packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-        packed.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-        *reinterpret_cast<uint2 *>(out + p * CO + mh * 16 + kq) = packed;
+#pragma unroll
+        for (int mh = 0; mh < MH; mh++) {
+            uint16_t v[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                float r = acc[mh][t];
+                if (MODE == MODE_RELU) {
+                    const int co = mh * 16 + kq + t;
+                    r += bias[co];
+                    r = r > 0.0f ? r : 0.0f;
+                }
+                v[t] = f2bf(r);
+            }
+            uint2 packed;
+            packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+            packed.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+            *reinterpret_cast<uint2 *>(out + p * CO + mh * 16 + kq) = packed;
+        }
     }
 }
 
@@ -390,8 +399,8 @@ template <int CI, int CO, int MODE>
 void conv(artefact_cnn *net, const uint16_t *in, const bf16x4 *w, const float *bias, uint16_t *out, float *pred,
           const float *resid, int B, int H, int W, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_conv<CI, CO, MODE>), dim3((W + 63) / 64, H, B), dim3(CT), 0, s, in, w, bias, out, pred,
-                       resid, H, W);
+    hipLaunchKernelGGL((k_conv<CI, CO, MODE>), dim3((W + 64 * NSEG - 1) / (64 * NSEG), H, B), dim3(CT), 0, s, in, w,
+                       bias, out, pred, resid, H, W);
 }
 
 // Forward pass on the features in act[0] / resid; the prediction (feature units) in pred.

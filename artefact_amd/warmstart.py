@@ -135,7 +135,7 @@ def qstep(coef) -> float:
 
 
 def training_pairs(count: int, size: int = 256, seed: int = 0, target_iterations: int = 300,
-                   qualities=(20, 90), weight: float = 0.3, pweight: float = 0.001):
+                   qualities=(20, 90), weight: float = 0.3, pweight: float = 0.001, log=print):
     """``count`` synthetic 4:2:0 crops of size x size at qualities drawn uniformly in the
     range; every channel plane (grid resolution) is one sample.  Returns (x0, q, target) with
     x0 the decoded start plane and target the plain TGV loop's output, all float32."""
@@ -149,9 +149,13 @@ def training_pairs(count: int, size: int = 256, seed: int = 0, target_iterations
             plan.init()
             starts += plan.download()
         qs += [qstep(c) for c in coefs]
-    with Batch(imgs, streams=4, tune=False) as b:
-        b.run()
-        targets = [p for i in range(count) for p in b.download(i)]
+        if log and (i + 1) % 512 == 0:
+            log(f"encoded {i + 1} of {count} crops", flush=True)
+    targets = []
+    for c0 in range(0, count, 64):  # graphs of at most 64 images
+        with Batch(imgs[c0:c0 + 64], streams=4, tune=False) as b:
+            b.run()
+            targets += [p for i in range(len(imgs[c0:c0 + 64])) for p in b.download(i)]
     shape = (len(starts), size, size)
     return (np.stack(starts).reshape(shape), np.asarray(qs, np.float32), np.stack(targets).reshape(shape))
 
@@ -182,9 +186,11 @@ class WarmResult:
 
 def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective: float | None = None,
                weight: float = 0.3, pweight=(0.001, 0.001, 0.001), plain_iterations: int = 100) -> WarmResult:
-    """CNN prediction -> projection -> ``iterations`` plain iterations; if the result's objective
-    is higher than the plain ``plain_iterations`` result's, the plain result is returned."""
+    """CNN prediction -> projection -> ``iterations`` plain iterations with the step of the
+    ``plain_iterations`` schedule; if the result's objective is higher than the plain
+    ``plain_iterations`` result's, the plain result is returned."""
     with Plan(coefs, weight, list(pweight), iterations, W, H) as plan:
+        _lib.check(plan._lib.artefact_hip_plan_set_step_iterations(plan._h, plain_iterations), "step")
         _lib.check(plan._lib.artefact_hip_plan_warm_init(plan._h, net._h), "artefact_hip_plan_warm_init")
         plan.iterate()
         planes = plan.download()
@@ -205,3 +211,39 @@ def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective:
             plan.iterate()
             plain = plan.download()
     return WarmResult(plain, plain_objective, True, fw)
+
+
+def main(argv=None) -> int:
+    """python -m artefact_amd.warmstart --crops N --steps S --out weights.npz"""
+    import argparse
+
+    ap = argparse.ArgumentParser(description="train the warm-start CNN on solver-generated pairs")
+    ap.add_argument("--crops", type=int, default=1024)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--target-iterations", type=int, default=300)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=WEIGHTS_FILE)
+    args = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    x0, q, target = training_pairs(args.crops, args.size, args.seed, args.target_iterations)
+    t_data = time.perf_counter() - t0
+    print(f"pairs: {x0.shape[0]} planes of {args.size}x{args.size} in {t_data:.1f} s", flush=True)
+    net = WarmStartNet(init_params(args.seed))
+    t0 = time.perf_counter()
+    losses = train(net, x0, q, target, args.steps, args.batch, args.lr, args.seed)
+    t_train = time.perf_counter() - t0
+    base = float(np.mean(((x0 - target) / 64.0) ** 2))
+    print(f"trained {args.steps} steps in {t_train:.1f} s; final loss {np.mean(losses[-50:]):.6g} "
+          f"(identity map: {base:.6g})", flush=True)
+    net.save(args.out, crops=args.crops, size=args.size, steps=args.steps, batch=args.batch, lr=args.lr,
+             target_iterations=args.target_iterations, seed=args.seed, data_seconds=t_data,
+             train_seconds=t_train, final_loss=float(np.mean(losses[-50:])), identity_loss=base)
+    net.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

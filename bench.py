@@ -217,6 +217,123 @@ def bench_precision(args) -> int:
     return 0
 
 
+def bench_warmstart(args) -> int:
+    """Learned warm start on 2048x1536 4:2:0 IJG q50 TGV: CNN prediction -> projection -> K plain
+    iterations, against the plain 100-iteration loop (time and objective), with the plain
+    fallback, and PSNR of each result against the FP64 loop run to convergence."""
+    import numpy as np
+
+    import artefact_amd
+    import oracle
+    from artefact_amd import synth, warmstart
+
+    W, H, K = 2048, 1536, args.warm_iterations
+    coefs = synth.encode(synth.synth_rgb(H, W, 0), subsampling="420", quality=50)
+    pw = [0.001] * 3
+    net = warmstart.WarmStartNet.load(args.weights)
+    meta = dict(np.load(args.weights, allow_pickle=False))
+    print(f"bench: warm start {W}x{H} q50, K={K}", file=sys.stderr, flush=True)
+
+    def timed(plan, start):
+        for _ in range(args.warmup):
+            start(plan)
+            plan.iterate()
+        plan.last_norms()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            start(plan)
+            plan.iterate()
+        plan.last_norms()
+        return 1e3 * (time.perf_counter() - t0) / args.steps
+
+    plain = artefact_amd.Plan(coefs, 0.3, pw, 100, W, H)
+    t_plain = timed(plain, lambda p: p.init())
+    plain.init()
+    plain.iterate()
+    f_plain, _ = plain.objective()
+    x_plain = plain.download()
+    plain.close()
+
+    def warm_start(p):
+        artefact_amd._lib.check(p._lib.artefact_hip_plan_warm_init(p._h, net._h), "artefact_hip_plan_warm_init")
+
+    warm = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
+    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, 100), "step")
+    t_warm = timed(warm, warm_start)
+    warm_start(warm)
+    warm.iterate()
+    f_warm, _ = warm.objective()
+    x_warm = warm.download()
+    warm.close()
+
+    cold = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
+    t_cold = timed(cold, lambda p: p.init())
+    cold.init()
+    cold.iterate()
+    f_cold, _ = cold.objective()
+    x_cold = cold.download()
+    cold.init()
+    x_dec = cold.download()
+    cold.close()
+    sweep = {}
+    for k in args.warm_sweep:
+        for kind in ("warm", "plain", "warm_step100"):
+            pl = artefact_amd.Plan(coefs, 0.3, pw, k, W, H)
+            if kind == "warm_step100":
+                artefact_amd._lib.check(pl._lib.artefact_hip_plan_set_step_iterations(pl._h, 100), "step")
+            t_k = timed(pl, warm_start if kind.startswith("warm") else (lambda q: q.init()))
+            (warm_start(pl) if kind.startswith("warm") else pl.init())
+            pl.iterate()
+            f_k, _ = pl.objective()
+            pl.close()
+            sweep[f"{kind}_{k}"] = {"ms": round(t_k, 4), "objective_rel_to_plain_100": float(f"{(f_k - f_plain) / f_plain:.4g}")}
+    fallback = f_warm > f_plain
+    print(f"bench: plain {t_plain:.3f} ms F={f_plain:.6g}; warm {t_warm:.3f} ms F={f_warm:.6g}; "
+          f"plain {K} {t_cold:.3f} ms F={f_cold:.6g}", file=sys.stderr, flush=True)
+    psnr = None
+    if not args.no_error:
+        print(f"bench: FP64 oracle, {args.converged_iterations} iterations", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        ref = oracle.compute_f64(coefs, 0.3, pw, args.converged_iterations, W, H)
+
+        def p(x):
+            mse = np.mean([np.mean((a.astype(np.float64) - r) ** 2) for a, r in zip(x, ref)])
+            return round(float(10 * np.log10(255.0 ** 2 / mse)), 3)
+
+        psnr = {"decoded": p(x_dec), "plain_100": p(x_plain), f"warm_cnn_plus_{K}": p(x_warm),
+                f"plain_{K}": p(x_cold), "fp64_seconds": round(time.perf_counter() - t0, 1),
+                "fp64_iterations": args.converged_iterations}
+    work = W * H * 100
+    res = {
+        "metric": "Mpixel*iterations/s + time to plain-100-iter objective",
+        "value": round(work / ((t_warm + (t_plain if fallback else 0.0)) * 1e-3) / 1e6, 2),
+        "unit": "Mpixel*it/s (plain-100-equivalent work over the warm-started time)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(t_warm + (t_plain if fallback else 0.0), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 loop, bf16 CNN (f32 accumulation)",
+        "data": "synthetic (seeded generator, artefact_amd/synth.py)",
+        "config": {
+            "workload": f"learned warm start: TGV 4:2:0 {W}x{H} IJG q50, CNN + {K} plain iterations (step of the "
+                        f"100-iteration schedule) vs plain 100",
+            "plain_100_ms": round(t_plain, 4), "plain_100_objective": f_plain,
+            f"warm_ms": round(t_warm, 4), "warm_objective": f_warm,
+            "warm_time_fraction_of_plain": round(t_warm / t_plain, 4),
+            "warm_objective_rel_to_plain": float(f"{(f_warm - f_plain) / f_plain:.4g}"),
+            "reaches_plain_objective_within_0.1pct": bool(f_warm <= f_plain * 1.001),
+            "used_fallback": bool(fallback),
+            f"plain_{K}_ms": round(t_cold, 4), f"plain_{K}_objective": f_cold,
+            "psnr_vs_fp64_converged_db": psnr,
+            "iteration_sweep": sweep,
+            "training": {k: v.item() if hasattr(v, "item") and np.asarray(v).ndim == 0 else None
+                         for k, v in meta.items() if k != "params"},
+        },
+    }
+    net.close()
+    print(json.dumps(res), flush=True)
+    return 0
+
+
 def bench_batch(args) -> int:
     """The batched form: ``batchN`` = N images (synth.batch_workload) in one graph."""
     import artefact_amd
@@ -310,7 +427,11 @@ def main() -> int:
     ap.add_argument("--workload", default="tgv_420_2048")
     ap.add_argument("--cpu-iters", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-error", action="store_true", help="precision workload: skip the FP64 oracle run")
+    ap.add_argument("--no-error", action="store_true", help="precision / warmstart: skip the FP64 oracle run")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "artefact_amd", "warmstart_weights.npz"))
+    ap.add_argument("--warm-iterations", type=int, default=15)
+    ap.add_argument("--converged-iterations", type=int, default=300)
+    ap.add_argument("--warm-sweep", type=int, nargs="*", default=[15, 25, 40, 60])
     ap.add_argument("--copy-sweep", action="store_true", help="also time 16/64/128 MiB copies")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print per-kernel HBM bytes from two rocprofv3 --pmc CSVs and exit")
@@ -332,6 +453,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "precision":
         return bench_precision(args)
+    if args.workload == "warmstart":
+        return bench_warmstart(args)
 
     wl = synth.workload(args.workload, seed=0)
     W, H, iters = wl["max_w"], wl["max_h"], wl["iterations"]

@@ -226,6 +226,10 @@ void artefact_cnn_destroy(artefact_cnn *net);
  * image as both x and x_prev.  artefact_hip_plan_iterate then runs the plan's iterations
  * from there. */
 int artefact_hip_plan_warm_init(artefact_hip_plan *plan, artefact_cnn *net);
+/* The iteration count the constant step is derived from: step = sqrt(N)/2/sqrt(1 + n)
+ * (pipeline_scalar/mod.rs:45,64-74 uses the plan's own iteration count; 0 restores that).
+ * A warm-started refinement of K iterations can take the step of a longer schedule. */
+int artefact_hip_plan_set_step_iterations(artefact_hip_plan *plan, uint32_t n);
 
 /* ---- JPEG input: Jpeg::from (backend/artefact-lib/jpeg/zune.rs:5-62) on the host ----
  * Decodes baseline/extended sequential and progressive Huffman JPEGs (8-bit, 1-4 components,

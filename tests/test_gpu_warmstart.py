@@ -1,0 +1,74 @@
+"""Learned warm start with the committed network (artefact_amd/warmstart_weights.npz, trained
+on the GPU by `python -m artefact_amd.warmstart` on 4096 solver-generated 256x256 crops).
+
+What the network gains: on crops outside its training set, K plain iterations from the
+CNN's projected prediction end at a lower objective and closer to the converged (300
+iteration) result than K plain iterations from the decoded image.  The plain solver stays the
+judge: warm_solve falls back to the plain result when the warm one's objective is higher.
+"""
+import numpy as np
+import pytest
+
+import artefact_amd
+from artefact_amd import synth, warmstart
+
+pytestmark = pytest.mark.gpu
+
+
+def crop(seed, quality, size=256):
+    return synth.encode(synth.synth_rgb(size, size, seed), subsampling="420", quality=quality)
+
+
+def solve(coefs, size, iters, net=None, step_iterations=0):
+    with artefact_amd.Plan(coefs, 0.3, [0.001] * 3, iters, size, size) as plan:
+        if step_iterations:
+            artefact_amd._lib.check(plan._lib.artefact_hip_plan_set_step_iterations(plan._h, step_iterations), "step")
+        if net is None:
+            plan.init()
+        else:
+            artefact_amd._lib.check(plan._lib.artefact_hip_plan_warm_init(plan._h, net._h), "warm")
+        plan.iterate()
+        f, _ = plan.objective()
+        return plan.download(), f
+
+
+@pytest.mark.parametrize("seed,quality", [(900001, 35), (900002, 60), (900003, 80)])
+def test_network_beats_the_same_iterations_from_the_decode(seed, quality):
+    net = warmstart.WarmStartNet.load()
+    coefs = crop(seed, quality)
+    print(f"case warm start seed={seed} q={quality}", flush=True)
+    ref, _ = solve(coefs, 256, 300)
+    cold, f_cold = solve(coefs, 256, 15)
+    warm, f_warm = solve(coefs, 256, 15, net)
+    net.close()
+    d_cold = np.sqrt(np.mean([np.mean((a - r) ** 2) for a, r in zip(cold, ref)]))
+    d_warm = np.sqrt(np.mean([np.mean((a - r) ** 2) for a, r in zip(warm, ref)]))
+    print(f"objective cold {f_cold:.6g} warm {f_warm:.6g}; rms to 300-iteration result cold {d_cold:.4g} warm {d_warm:.4g}")
+    assert f_warm < f_cold
+    assert d_warm < d_cold
+
+
+def test_untrained_network_gains_less():
+    """The gain comes from training: an untrained (near-identity) network is worse than the
+    trained one after the same iterations."""
+    coefs = crop(900004, 50)
+    trained = warmstart.WarmStartNet.load()
+    untrained = warmstart.WarmStartNet(warmstart.init_params(0))
+    _, f_t = solve(coefs, 256, 15, trained)
+    _, f_u = solve(coefs, 256, 15, untrained)
+    trained.close()
+    untrained.close()
+    assert f_t < f_u
+
+
+def test_fallback_returns_the_plain_result():
+    coefs = crop(900005, 50, size=128)
+    net = warmstart.WarmStartNet.load()
+    res = warmstart.warm_solve(coefs, 128, 128, net, iterations=3, plain_iterations=50)
+    net.close()
+    plain, f_plain = solve(coefs, 128, 50)
+    assert res.used_fallback == (res.warm_objective > f_plain)
+    if res.used_fallback:
+        for a, b in zip(res.planes, plain):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert res.objective <= f_plain
