@@ -146,7 +146,7 @@ class Plan:
         """Autotunes strip height and projection kernel for this geometry; re-init after."""
         t = _lib.artefact_tuning()
         _lib.check(self._lib.artefact_hip_plan_tune(self._h, ctypes.byref(t)), "artefact_hip_plan_tune")
-        return {"kg_rows": t.kg_rows, "project_kernel": "fused" if t.project_kernel else "per-channel",
+        return {"kg_rows": t.kg_rows, "project_kernel": {0: "per-channel", 1: "fused-lds", 2: "fused-lane"}[t.project_kernel],
                 "ms_per_iteration": t.ms_per_iteration}
 
     def download_rgb(self, real_w: int, real_h: int) -> np.ndarray:

@@ -1498,6 +1498,202 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// k_project_lane: the same projection as k_project_fused (same Ooura passes on the same
+// vectors, so the same bits) without LDS or barriers.  Eight lanes own one 8x8 block, lane j
+// its row j: rows load and store coalesced, and the column passes run on columns obtained
+// by an 8x8 transpose across the group's lanes (three xor-exchange stages: DPP quad_perm for
+// xor 1 and 2, ds_swizzle for xor 4).  A wave takes 8 consecutive blocks of one block row of
+// one channel; chroma blocks of 4:2:0 take their 2x2 means in registers.
+template <int S>
+__device__ __forceinline__ float lane_xor(float v)
+{
+    const int b = __float_as_int(v);
+    if constexpr (S == 1) return __int_as_float(__builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, false));
+    else if constexpr (S == 2) return __int_as_float(__builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, false));
+    else return __int_as_float(__builtin_amdgcn_ds_swizzle(b, 0x101F));  // bitmask mode, xor 4
+}
+
+// Lane j holds a[k] = M[j][k]; afterwards a[k] = M[k][j].
+template <int S>
+__device__ __forceinline__ void transpose_stage(float a[8], bool hi)
+{
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (k & S) continue;
+        const float send = hi ? a[k] : a[k + S];
+        const float recv = lane_xor<S>(send);
+        if (hi) a[k] = recv;
+        else a[k + S] = recv;
+    }
+}
+__device__ __forceinline__ void transpose8(float a[8], int j)
+{
+    transpose_stage<4>(a, (j & 4) != 0);
+    transpose_stage<2>(a, (j & 2) != 0);
+    transpose_stage<1>(a, (j & 1) != 0);
+}
+
+// DCT of the block whose row j this lane holds (column pass, then row pass: dct.rs:100-169),
+// the clamp and the prob difference for coefficient row j, then the IDCTs of both (column
+// pass, then row pass: dct.rs:30-98).  On return px holds row j of the projected block and
+// pr row j of the prob block (before alpha_p).
+__device__ __forceinline__ void lane_project_block(float v[8], int j, const ChanDev &ch, size_t blk, bool valid,
+                                                   bool write_cos, float pr[8])
+{
+    transpose8(v, j);
+    dct8(v);  // column j
+    transpose8(v, j);
+    dct8(v);  // row j: coefficients (u = j, v = k)
+    float cf[8];
+    if (valid) {
+        const int4 raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
+        const int w4[4] = {raw.x, raw.y, raw.z, raw.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cf[2 * k] = (float)(short)(w4[k] & 0xFFFF);
+            cf[2 * k + 1] = (float)(short)((uint32_t)w4[k] >> 16);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; k++) cf[k] = 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float q = ch.quant[j * 8 + k];
+        const float o = clamp_ref(v[k], (cf[k] - 0.5f) * q, (cf[k] + 0.5f) * q);
+        pr[k] = qdiv(o - cf[k] * q, ch.rqq[j * 8 + k]);
+        v[k] = o;
+    }
+    if (write_cos && valid) {
+        float *c = ch.cos + blk * 64 + j * 8;
+        *reinterpret_cast<float4 *>(c) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4 *>(c + 4) = make_float4(v[4], v[5], v[6], v[7]);
+    }
+    transpose8(v, j);
+    transpose8(pr, j);
+    idct8(v);
+    idct8(pr);
+    transpose8(v, j);
+    transpose8(pr, j);
+    idct8(v);
+    idct8(pr);
+}
+
+template <int MODE>
+__device__ __forceinline__ void lane_unit(const IterArgs &a, int c, int by, int bx, int j, bool valid)
+{
+    const ChanDev &ch = a.ch[c];
+    const int W = a.W;
+    const float norm = a.norms[c];
+    const double rn = a.rnorm[c];
+    const float step = a.step;
+    const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
+    auto descend = [&](float4 yv, float4 gv, float out[4]) {
+        const float ys[4] = {yv.x, yv.y, yv.z, yv.w};
+        const float gs[4] = {gv.x, gv.y, gv.z, gv.w};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float y = ys[k];
+            if (norm != 0.0f) y -= step * qdiv(gs[k], rn);  // compute_step.rs:61-65
+            out[k] = y;
+        }
+    };
+    float v[8], pr[8];
+    if constexpr (MODE == PM_FULL) {
+        const size_t p = valid ? (size_t)(by * 8 + j) * W + bx * 8 : 0;
+        if (valid) {
+            descend(*reinterpret_cast<const float4 *>(a.y[c] + p), *reinterpret_cast<const float4 *>(a.g[c] + p), v);
+            descend(*reinterpret_cast<const float4 *>(a.y[c] + p + 4),
+                    *reinterpret_cast<const float4 *>(a.g[c] + p + 4), v + 4);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = 0.0f;
+        }
+        lane_project_block(v, j, ch, blk, valid, a.write_cos != 0, pr);
+        if (valid) {
+            *reinterpret_cast<float4 *>(a.xnew[c] + p) = make_float4(v[0], v[1], v[2], v[3]);
+            *reinterpret_cast<float4 *>(a.xnew[c] + p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+        }
+    } else {  // PM_SUB22: row j of the means block = full-resolution rows 2j, 2j+1, 16 columns
+        float yd[2][16];
+#pragma unroll
+        for (int sy = 0; sy < 2; sy++) {
+            const size_t p = valid ? (size_t)(by * 16 + 2 * j + sy) * W + bx * 16 : 0;
+#pragma unroll
+            for (int q4 = 0; q4 < 4; q4++) {
+                if (valid)
+                    descend(*reinterpret_cast<const float4 *>(a.y[c] + p + 4 * q4),
+                            *reinterpret_cast<const float4 *>(a.g[c] + p + 4 * q4), &yd[sy][4 * q4]);
+                else
+#pragma unroll
+                    for (int k = 0; k < 4; k++) yd[sy][4 * q4 + k] = 0.0f;
+            }
+        }
+        // Box means in the reference's order (compute_projection.rs:21-31), then the residual.
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float m = 0.0f;
+            m += yd[0][2 * k];
+            m += yd[0][2 * k + 1];
+            m += yd[1][2 * k];
+            m += yd[1][2 * k + 1];
+            m /= 4.0f;
+            v[k] = m;
+#pragma unroll
+            for (int sy = 0; sy < 2; sy++) {
+                yd[sy][2 * k] -= m;
+                yd[sy][2 * k + 1] -= m;
+            }
+        }
+        lane_project_block(v, j, ch, blk, valid, a.write_cos != 0, pr);
+        if (valid) {
+#pragma unroll
+            for (int sy = 0; sy < 2; sy++) {
+                float o[16];
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    o[2 * k] = yd[sy][2 * k] + v[k];
+                    o[2 * k + 1] = yd[sy][2 * k + 1] + v[k];
+                }
+                float *dst = a.xnew[c] + (size_t)(by * 16 + 2 * j + sy) * W + bx * 16;
+#pragma unroll
+                for (int q4 = 0; q4 < 4; q4++)
+                    *reinterpret_cast<float4 *>(dst + 4 * q4) =
+                        make_float4(o[4 * q4], o[4 * q4 + 1], o[4 * q4 + 2], o[4 * q4 + 3]);
+            }
+        }
+    }
+    // Next iteration's prob plane, component raster (compute_step_prob.rs:24-55 for x_new).
+    if (valid && ch.alpha_p != 0.0f) {
+        const float ap = ch.alpha_p;
+        float *row = ch.prob + (size_t)(by * 8 + j) * ch.rw + bx * 8;
+        *reinterpret_cast<float4 *>(row) = make_float4(ap * pr[0], ap * pr[1], ap * pr[2], ap * pr[3]);
+        *reinterpret_cast<float4 *>(row + 4) = make_float4(ap * pr[4], ap * pr[5], ap * pr[6], ap * pr[7]);
+    }
+}
+
+// One wave per unit = 8 consecutive blocks of a block row; units of channel 0 first.  units[c]
+// is the first unit of channel c + 1 (cumulative).
+template <int C, int M0, int M1, int M2>
+__global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t u0, uint32_t u1, uint32_t u2)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
+    if (unit >= u2) return;
+    const int g = lane >> 3, j = lane & 7;
+    auto run = [&](auto MODE_C, int c, uint32_t local) {
+        constexpr int MODE = decltype(MODE_C)::value;
+        const ChanDev &ch = a.ch[c];
+        const uint32_t per_row = (ch.bw + 7) / 8;
+        const int by = (int)(local / per_row), bx = (int)(local % per_row) * 8 + g;
+        lane_unit<MODE>(a, c, by, bx, j, bx < (int)ch.bw);
+    };
+    if (unit < u0) run(IC<M0>{}, 0, unit);
+    else if (C > 1 && unit < u1) run(IC<M1>{}, 1, unit - u0);
+    else if (C > 2) run(IC<M2>{}, 2, unit - u1);
+}
+
 }  // namespace
 
 struct artefact_hip_plan {
@@ -1511,6 +1707,8 @@ struct artefact_hip_plan {
     int rows = 8;  // k_gradient rows per wave
     int layout = 0;  // LAYOUT_*: which k_project variant runs
     int layout_fused = 0;  // the fused layout this plan's channels allow (LAYOUT_GENERIC if none)
+    int lane_project = 0;  // fused layouts: 1 = k_project_lane (no LDS), 0 = k_project_fused
+    uint32_t lane_units[3] = {0, 0, 0};  // k_project_lane: cumulative units (8-block row pieces) per channel
     uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
     float qmean[3] = {0, 0, 0};                   // mean quantisation step per channel (warm start)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
@@ -1641,6 +1839,21 @@ void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t 
         else hipLaunchKernelGGL(k_project_mfma<3>, pl->grid_p, dim3(NT), 0, s, a);
         return;
     }
+    if (pl->lane_project && pl->layout != LAYOUT_GENERIC) {
+        const dim3 grid((pl->lane_units[2] + NT / 64 - 1) / (NT / 64));
+        const uint32_t u0 = pl->lane_units[0], u1 = pl->lane_units[1], u2 = pl->lane_units[2];
+        switch (pl->layout) {
+        case LAYOUT_Y:
+            hipLaunchKernelGGL((k_project_lane<1, PM_FULL, PM_FULL, PM_FULL>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            return;
+        case LAYOUT_444:
+            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_FULL, PM_FULL>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            return;
+        default:
+            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_SUB22, PM_SUB22>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            return;
+        }
+    }
     switch (pl->layout) {
     case LAYOUT_Y:
         hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
@@ -1745,7 +1958,7 @@ int run_colour(hipStream_t stream, const float *y, const float *cb, const float 
 // Candidates are timed on the plan itself (eager launches, HIP events around a few
 // iterations after a fresh init) and the winner is cached for the process.
 struct TuneChoice {
-    int rows, layout;
+    int rows, layout, lane;
     float ms;
 };
 std::mutex g_tune_mu;
@@ -1803,7 +2016,7 @@ void drop_graph(artefact_hip_plan *pl)
 int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
 {
     const std::string key = tune_key(pl);
-    TuneChoice best{pl->rows, pl->layout, -1.0f};
+    TuneChoice best{pl->rows, pl->layout, pl->lane_project, -1.0f};
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -1815,23 +2028,28 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
         if (!getenv("ARTEFACT_KG_ROWS"))
             for (int r : {r0 + 1, r0 + 2, r0 - 1, r0 - 2, 2 * r0})
                 if (r >= 2 && r <= (int)pl->H && std::find(rows.begin(), rows.end(), r) == rows.end()) rows.push_back(r);
-        std::vector<int> layouts{pl->layout};
-        if (pl->layout != LAYOUT_GENERIC) layouts.push_back(LAYOUT_GENERIC);
+        // Projection candidates: (layout, lane kernel) -- the fused layout with the LDS-free and
+        // the LDS kernel, and the per-channel kernel.
+        std::vector<std::pair<int, int>> projs{{pl->layout, pl->lane_project}};
+        if (pl->layout != LAYOUT_GENERIC && !getenv("ARTEFACT_PROJECT_KERNEL")) projs.push_back({pl->layout, 1 - pl->lane_project});
+        if (pl->layout != LAYOUT_GENERIC) projs.push_back({LAYOUT_GENERIC, 0});
         const uint32_t nt = std::min<uint32_t>(std::max<uint32_t>(pl->iterations, 2), 8);
-        for (int lay : layouts)
+        for (const auto &pj : projs)
             for (int r : rows) {
-                pl->layout = lay;
+                pl->layout = pj.first;
+                pl->lane_project = pj.second;
                 set_rows(pl, r);
                 float t = time_loop(pl, nt);  // first pass warms caches and code
                 if (t < 0.0f) return ARTEFACT_E_HIP;
                 t = std::min(t, time_loop(pl, nt));
                 if (t < 0.0f) return ARTEFACT_E_HIP;
-                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, lay, t};
+                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, pj.first, pj.second, t};
             }
         std::lock_guard<std::mutex> lock(g_tune_mu);
         g_tune_cache[key] = best;
     }
     pl->layout = best.layout;
+    pl->lane_project = best.lane;
     set_rows(pl, best.rows);
     drop_graph(pl);
     pl->initialized = false;
@@ -1981,6 +2199,17 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         pl->layout = LAYOUT_420;
     else pl->layout = LAYOUT_GENERIC;
     pl->layout_fused = pl->layout;
+    {
+        uint32_t acc = 0;
+        for (uint32_t c = 0; c < 3; c++) {
+            if (c < pl->C) acc += ((pl->ch[c].bw + 7) / 8) * pl->ch[c].bh;
+            pl->lane_units[c] = acc;
+        }
+    }
+    // Untuned default, from measurement on MI355X: the LDS-free kernel for full-resolution
+    // layouts (4096x3072 4:4:4: 144 vs 194 us), the LDS kernel for 4:2:0 (2048x1536: 36 vs 41 us).
+    pl->lane_project = pl->layout == LAYOUT_420 ? 0 : 1;
+    if (const char *e = getenv("ARTEFACT_PROJECT_KERNEL")) pl->lane_project = strcmp(e, "lds") != 0;
     if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
         if (atoi(e)) pl->layout = LAYOUT_GENERIC;
     ok = ok && hipMemcpyAsync(pl->quant, quant.data(), quant.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -2345,12 +2574,12 @@ int artefact_hip_plan_tune(artefact_hip_plan *pl, artefact_tuning *out)
 {
     if (!pl) return ARTEFACT_E_ARGUMENT;
     DeviceGuard guard(pl->device);
-    TuneChoice t{pl->rows, pl->layout, -1.0f};
+    TuneChoice t{pl->rows, pl->layout, pl->lane_project, -1.0f};
     const int rc = tune_plan(pl, &t);
     if (rc) return rc;
     if (out) {
         out->kg_rows = (uint32_t)t.rows;
-        out->project_kernel = t.layout == LAYOUT_GENERIC ? 0u : 1u;
+        out->project_kernel = t.layout == LAYOUT_GENERIC ? 0u : (t.lane ? 2u : 1u);
         out->ms_per_iteration = t.ms;
     }
     return ARTEFACT_OK;

@@ -125,8 +125,9 @@ void artefact_hip_plan_destroy(artefact_hip_plan *plan);
 int artefact_hip_plan_set_precision(artefact_hip_plan *plan, uint32_t precision);
 
 /* Autotuner result: the k_gradient strip height (output rows per wave) and the projection
- * kernel (1 = the fused multi-channel kernel, 0 = the per-channel kernel) chosen for the
- * plan's geometry, and the measured milliseconds per iteration. */
+ * kernel (2 = the fused multi-channel kernel without LDS, 1 = the fused kernel with LDS
+ * transposes, 0 = the per-channel kernel) chosen for the plan's geometry, and the measured
+ * milliseconds per iteration. */
 typedef struct {
     uint32_t kg_rows;
     uint32_t project_kernel;

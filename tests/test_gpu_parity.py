@@ -233,3 +233,19 @@ def test_first_gradient_extreme_scales(scale):
     gref, _, _ = oracle.gradient(wl["coefs"], 0.3, wl["pweight"], W, H, planes, cos)
     for c, (a, b) in enumerate(zip(g, gref)):
         assert same_bits_or_both_nan(a, b), f"channel {c}"
+
+
+@pytest.mark.parametrize("kernel", ["lds", "lane"])
+@pytest.mark.parametrize("name,iters", [("tgv_420_208x144", 6), ("tgv_444_72x40", 5), ("tv_luma_64x64", 4),
+                                        ("tgv_420_96x64", 1)])
+def test_both_fused_projection_kernels_bit_exact(kernel, name, iters, monkeypatch):
+    """The fused layouts have two projection kernels (LDS transposes / lane transposes); both
+    give the oracle's bits."""
+    monkeypatch.setenv("ARTEFACT_PROJECT_KERNEL", kernel)
+    wl = synth.workload(name, seed=23)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case {name} iters={iters} kernel={kernel}", flush=True)
+    gpu = run_gpu(wl, iters)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, norm_mode="f64")
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
