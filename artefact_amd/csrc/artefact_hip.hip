@@ -1500,45 +1500,66 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
 
 // ---------------------------------------------------------------------------------------
 // k_project_lane: the same projection as k_project_fused (same Ooura passes on the same
-// vectors, so the same bits) without LDS or barriers.  Eight lanes own one 8x8 block, lane j
-// its row j: rows load and store coalesced, and the column passes run on columns obtained
-// by an 8x8 transpose across the group's lanes (three xor-exchange stages: DPP quad_perm for
-// xor 1 and 2, ds_swizzle for xor 4).  A wave takes 8 consecutive blocks of one block row of
-// one channel; chroma blocks of 4:2:0 take their 2x2 means in registers.
-template <int S>
+// vectors, so the same bits) without LDS or barriers.  Eight lanes own one 8x8 block, one
+// block row each, so rows load and store coalesced; the column passes run on columns obtained
+// by an 8x8 transpose across those lanes.  The block-row index j sits in lane bits 3-5, which
+// makes two of the transpose's three exchange stages plain register swaps between lane halves
+// (v_permlane32_swap, j bit 2) and between 16-lane rows (v_permlane16_swap, j bit 1); only the
+// j bit 0 stage (lane ^ 8, DPP row_ror:8) selects.  A wave takes 8 consecutive blocks of one
+// block row of a full-resolution channel (lane = 8 j + block), or 4 blocks of 4:2:0 chroma
+// (lane = 4 r + block for full-resolution row r = 2 j + sy of the 16-row band): the two lanes
+// of a block row exchange their pixel pairs (lane ^ 4) to form the 2x2 means, and both carry
+// that block row through the transforms.
+
+// Value of the same register in lane ^ X (X = 4: ds_swizzle bitmask mode; X = 8: DPP row_ror:8).
+template <int X>
 __device__ __forceinline__ float lane_xor(float v)
 {
     const int b = __float_as_int(v);
-    if constexpr (S == 1) return __int_as_float(__builtin_amdgcn_update_dpp(0, b, 0xB1, 0xF, 0xF, false));
-    else if constexpr (S == 2) return __int_as_float(__builtin_amdgcn_update_dpp(0, b, 0x4E, 0xF, 0xF, false));
-    else return __int_as_float(__builtin_amdgcn_ds_swizzle(b, 0x101F));  // bitmask mode, xor 4
+    static_assert(X == 4 || X == 8, "lane_xor: 4 or 8");
+    if constexpr (X == 8) return __int_as_float(__builtin_amdgcn_update_dpp(0, b, 0x128, 0xF, 0xF, false));
+    else return __int_as_float(__builtin_amdgcn_ds_swizzle(b, 0x101F));
 }
 
-// Lane j holds a[k] = M[j][k]; afterwards a[k] = M[k][j].
-template <int S>
-__device__ __forceinline__ void transpose_stage(float a[8], bool hi)
-{
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        if (k & S) continue;
-        const float send = hi ? a[k] : a[k + S];
-        const float recv = lane_xor<S>(send);
-        if (hi) a[k] = recv;
-        else a[k + S] = recv;
-    }
-}
+// Lane with block row j (lane bits 3-5) holds a[k] = M[j][k]; afterwards a[k] = M[k][j].
+// Stage S exchanges a[k + S] of the lanes with j & S == 0 against a[k] of their partners
+// (j ^ S) for every k with k & S == 0.
 __device__ __forceinline__ void transpose8(float a[8], int j)
 {
-    transpose_stage<4>(a, (j & 4) != 0);
-    transpose_stage<2>(a, (j & 2) != 0);
-    transpose_stage<1>(a, (j & 1) != 0);
+    // j bit 2 = lane bit 5: the upper 32 lanes of a[k] trade with the lower 32 of a[k + 4].
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a[k]), __float_as_uint(a[k + 4]), false, false);
+        a[k] = __uint_as_float(r[0]);
+        a[k + 4] = __uint_as_float(r[1]);
+    }
+    // j bit 1 = lane bit 4: the odd 16-lane rows of a[k] trade with the even rows of a[k + 2].
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        if (k & 2) continue;
+        const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a[k]), __float_as_uint(a[k + 2]), false, false);
+        a[k] = __uint_as_float(r[0]);
+        a[k + 2] = __uint_as_float(r[1]);
+    }
+    // j bit 0 = lane bit 3: lane ^ 8 with selects (on values read out first, so that they stay
+    // register selects and never become a lane-dependent index into the array).
+    const bool hi = (j & 1) != 0;
+#pragma unroll
+    for (int k = 0; k < 8; k += 2) {
+        const float lo_v = a[k], hi_v = a[k + 1];
+        const float recv = lane_xor<8>(hi ? lo_v : hi_v);
+        a[k] = hi ? recv : lo_v;
+        a[k + 1] = hi ? hi_v : recv;
+    }
 }
 
 // DCT of the block whose row j this lane holds (column pass, then row pass: dct.rs:100-169),
 // the clamp and the prob difference for coefficient row j, then the IDCTs of both (column
-// pass, then row pass: dct.rs:30-98).  On return px holds row j of the projected block and
-// pr row j of the prob block (before alpha_p).
-__device__ __forceinline__ void lane_project_block(float v[8], int j, const ChanDev &ch, size_t blk, bool valid,
+// pass, then row pass: dct.rs:30-98).  raw: the block's int16 coefficients of row j, loaded by
+// the caller.  On return v holds row j of the projected block and pr row j of the prob block
+// (before alpha_p).  `store` selects the lane that writes cos.
+__device__ __forceinline__ void lane_project_block(float v[8], int j, const ChanDev &ch, size_t blk, int4 raw,
+                                                   const float q8[8], const double rq8[8], bool valid, bool store,
                                                    bool write_cos, float pr[8])
 {
     transpose8(v, j);
@@ -1546,26 +1567,20 @@ __device__ __forceinline__ void lane_project_block(float v[8], int j, const Chan
     transpose8(v, j);
     dct8(v);  // row j: coefficients (u = j, v = k)
     float cf[8];
-    if (valid) {
-        const int4 raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
-        const int w4[4] = {raw.x, raw.y, raw.z, raw.w};
+    const int w4[4] = {raw.x, raw.y, raw.z, raw.w};
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            cf[2 * k] = (float)(short)(w4[k] & 0xFFFF);
-            cf[2 * k + 1] = (float)(short)((uint32_t)w4[k] >> 16);
-        }
-    } else {
-#pragma unroll
-        for (int k = 0; k < 8; k++) cf[k] = 0.0f;
+    for (int k = 0; k < 4; k++) {
+        cf[2 * k] = (float)(short)(w4[k] & 0xFFFF);
+        cf[2 * k + 1] = (float)(short)((uint32_t)w4[k] >> 16);
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        const float q = ch.quant[j * 8 + k];
+        const float q = q8[k];
         const float o = clamp_ref(v[k], (cf[k] - 0.5f) * q, (cf[k] + 0.5f) * q);
-        pr[k] = qdiv(o - cf[k] * q, ch.rqq[j * 8 + k]);
+        pr[k] = qdiv(o - cf[k] * q, rq8[k]);
         v[k] = o;
     }
-    if (write_cos && valid) {
+    if (write_cos && valid && store) {
         float *c = ch.cos + blk * 64 + j * 8;
         *reinterpret_cast<float4 *>(c) = make_float4(v[0], v[1], v[2], v[3]);
         *reinterpret_cast<float4 *>(c + 4) = make_float4(v[4], v[5], v[6], v[7]);
@@ -1580,118 +1595,150 @@ __device__ __forceinline__ void lane_project_block(float v[8], int j, const Chan
     idct8(pr);
 }
 
-template <int MODE>
-__device__ __forceinline__ void lane_unit(const IterArgs &a, int c, int by, int bx, int j, bool valid)
+__device__ __forceinline__ void lane_descend(float norm, double rn, float step, float4 yv, float4 gv, float out[4])
 {
-    const ChanDev &ch = a.ch[c];
-    const int W = a.W;
-    const float norm = a.norms[c];
-    const double rn = a.rnorm[c];
-    const float step = a.step;
-    const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
-    auto descend = [&](float4 yv, float4 gv, float out[4]) {
-        const float ys[4] = {yv.x, yv.y, yv.z, yv.w};
-        const float gs[4] = {gv.x, gv.y, gv.z, gv.w};
+    const float ys[4] = {yv.x, yv.y, yv.z, yv.w};
+    const float gs[4] = {gv.x, gv.y, gv.z, gv.w};
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float y = ys[k];
-            if (norm != 0.0f) y -= step * qdiv(gs[k], rn);  // compute_step.rs:61-65
-            out[k] = y;
-        }
-    };
-    float v[8], pr[8];
-    if constexpr (MODE == PM_FULL) {
-        const size_t p = valid ? (size_t)(by * 8 + j) * W + bx * 8 : 0;
-        if (valid) {
-            descend(*reinterpret_cast<const float4 *>(a.y[c] + p), *reinterpret_cast<const float4 *>(a.g[c] + p), v);
-            descend(*reinterpret_cast<const float4 *>(a.y[c] + p + 4),
-                    *reinterpret_cast<const float4 *>(a.g[c] + p + 4), v + 4);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = 0.0f;
-        }
-        lane_project_block(v, j, ch, blk, valid, a.write_cos != 0, pr);
-        if (valid) {
-            *reinterpret_cast<float4 *>(a.xnew[c] + p) = make_float4(v[0], v[1], v[2], v[3]);
-            *reinterpret_cast<float4 *>(a.xnew[c] + p + 4) = make_float4(v[4], v[5], v[6], v[7]);
-        }
-    } else {  // PM_SUB22: row j of the means block = full-resolution rows 2j, 2j+1, 16 columns
-        float yd[2][16];
-#pragma unroll
-        for (int sy = 0; sy < 2; sy++) {
-            const size_t p = valid ? (size_t)(by * 16 + 2 * j + sy) * W + bx * 16 : 0;
-#pragma unroll
-            for (int q4 = 0; q4 < 4; q4++) {
-                if (valid)
-                    descend(*reinterpret_cast<const float4 *>(a.y[c] + p + 4 * q4),
-                            *reinterpret_cast<const float4 *>(a.g[c] + p + 4 * q4), &yd[sy][4 * q4]);
-                else
-#pragma unroll
-                    for (int k = 0; k < 4; k++) yd[sy][4 * q4 + k] = 0.0f;
-            }
-        }
-        // Box means in the reference's order (compute_projection.rs:21-31), then the residual.
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float m = 0.0f;
-            m += yd[0][2 * k];
-            m += yd[0][2 * k + 1];
-            m += yd[1][2 * k];
-            m += yd[1][2 * k + 1];
-            m /= 4.0f;
-            v[k] = m;
-#pragma unroll
-            for (int sy = 0; sy < 2; sy++) {
-                yd[sy][2 * k] -= m;
-                yd[sy][2 * k + 1] -= m;
-            }
-        }
-        lane_project_block(v, j, ch, blk, valid, a.write_cos != 0, pr);
-        if (valid) {
-#pragma unroll
-            for (int sy = 0; sy < 2; sy++) {
-                float o[16];
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    o[2 * k] = yd[sy][2 * k] + v[k];
-                    o[2 * k + 1] = yd[sy][2 * k + 1] + v[k];
-                }
-                float *dst = a.xnew[c] + (size_t)(by * 16 + 2 * j + sy) * W + bx * 16;
-#pragma unroll
-                for (int q4 = 0; q4 < 4; q4++)
-                    *reinterpret_cast<float4 *>(dst + 4 * q4) =
-                        make_float4(o[4 * q4], o[4 * q4 + 1], o[4 * q4 + 2], o[4 * q4 + 3]);
-            }
-        }
-    }
-    // Next iteration's prob plane, component raster (compute_step_prob.rs:24-55 for x_new).
-    if (valid && ch.alpha_p != 0.0f) {
-        const float ap = ch.alpha_p;
-        float *row = ch.prob + (size_t)(by * 8 + j) * ch.rw + bx * 8;
-        *reinterpret_cast<float4 *>(row) = make_float4(ap * pr[0], ap * pr[1], ap * pr[2], ap * pr[3]);
-        *reinterpret_cast<float4 *>(row + 4) = make_float4(ap * pr[4], ap * pr[5], ap * pr[6], ap * pr[7]);
+    for (int k = 0; k < 4; k++) {
+        float y = ys[k];
+        if (norm != 0.0f) y -= step * qdiv(gs[k], rn);  // compute_step.rs:61-65
+        out[k] = y;
     }
 }
 
-// One wave per unit = 8 consecutive blocks of a block row; units of channel 0 first.  units[c]
-// is the first unit of channel c + 1 (cumulative).
+__device__ __forceinline__ void lane_store_prob(const ChanDev &ch, int by, int bx, int j, const float pr[8])
+{
+    // Next iteration's prob plane, component raster (compute_step_prob.rs:24-55 for x_new).
+    if (ch.alpha_p == 0.0f) return;
+    const float ap = ch.alpha_p;
+    float *row = ch.prob + (size_t)(by * 8 + j) * ch.rw + bx * 8;
+    *reinterpret_cast<float4 *>(row) = make_float4(ap * pr[0], ap * pr[1], ap * pr[2], ap * pr[3]);
+    *reinterpret_cast<float4 *>(row + 4) = make_float4(ap * pr[4], ap * pr[5], ap * pr[6], ap * pr[7]);
+}
+
+// Every load of a unit is issued up front and unconditionally: lanes of blocks past the
+// channel's width (valid false) read block 0 / pixel 0, run the transforms on those values
+// within their own lane group, and store nothing.
+
+// Full-resolution channel: 8 lanes per block, block row j = lane >> 3.
+__device__ __forceinline__ void lane_unit_full(const IterArgs &a, int c, int by, int bx, int j, bool valid)
+{
+    const ChanDev &ch = a.ch[c];
+    const float norm = a.norms[c];
+    const double rn = a.rnorm[c];
+    const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
+    const size_t p = valid ? (size_t)(by * 8 + j) * a.W + bx * 8 : 0;
+    const float4 *yp = reinterpret_cast<const float4 *>(a.y[c] + p);
+    const float4 *gp = reinterpret_cast<const float4 *>(a.g[c] + p);
+    const float4 y0 = yp[0], y1 = yp[1], g0 = gp[0], g1 = gp[1];
+    const int4 raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
+    float q8[8];
+    double rq8[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        q8[k] = ch.quant[j * 8 + k];
+        rq8[k] = ch.rqq[j * 8 + k];
+    }
+    float v[8], pr[8];
+    lane_descend(norm, rn, a.step, y0, g0, v);
+    lane_descend(norm, rn, a.step, y1, g1, v + 4);
+    lane_project_block(v, j, ch, blk, raw, q8, rq8, valid, true, a.write_cos != 0, pr);
+    if (valid) {
+        *reinterpret_cast<float4 *>(a.xnew[c] + p) = make_float4(v[0], v[1], v[2], v[3]);
+        *reinterpret_cast<float4 *>(a.xnew[c] + p + 4) = make_float4(v[4], v[5], v[6], v[7]);
+        lane_store_prob(ch, by, bx, j, pr);
+    }
+}
+
+// 4:2:0 chroma: 16 lanes per block, full-resolution row r = lane >> 2 of the 16-row band; block
+// row j = r / 2 is held twice (rows 2j and 2j + 1, lanes 4 apart), and the upper row's lane
+// stores the block's outputs.
+__device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, int c, int by, int bx, int r, bool valid)
+{
+    const ChanDev &ch = a.ch[c];
+    const float norm = a.norms[c];
+    const double rn = a.rnorm[c];
+    const int j = r >> 1, sy = r & 1;
+    const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
+    const size_t p = valid ? (size_t)(by * 16 + r) * a.W + bx * 16 : 0;
+    const float4 *yp = reinterpret_cast<const float4 *>(a.y[c] + p);
+    const float4 *gp = reinterpret_cast<const float4 *>(a.g[c] + p);
+    float4 yv[4], gv[4];
+#pragma unroll
+    for (int q4 = 0; q4 < 4; q4++) {
+        yv[q4] = yp[q4];
+        gv[q4] = gp[q4];
+    }
+    const int4 raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
+    float q8[8];
+    double rq8[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        q8[k] = ch.quant[j * 8 + k];
+        rq8[k] = ch.rqq[j * 8 + k];
+    }
+    float yd[16];
+#pragma unroll
+    for (int q4 = 0; q4 < 4; q4++) lane_descend(norm, rn, a.step, yv[q4], gv[q4], &yd[4 * q4]);
+    // Box means in the reference's order (compute_projection.rs:21-31): the upper row's pair,
+    // then the lower row's; both lanes of a block row form the same sums.
+    float v[8], pr[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float t0 = lane_xor<4>(yd[2 * k]), t1 = lane_xor<4>(yd[2 * k + 1]);
+        const float u0 = sy ? t0 : yd[2 * k], u1 = sy ? t1 : yd[2 * k + 1];
+        const float l0 = sy ? yd[2 * k] : t0, l1 = sy ? yd[2 * k + 1] : t1;
+        float m = 0.0f;
+        m += u0;
+        m += u1;
+        m += l0;
+        m += l1;
+        m /= 4.0f;
+        v[k] = m;
+        yd[2 * k] -= m;
+        yd[2 * k + 1] -= m;
+    }
+    lane_project_block(v, j, ch, blk, raw, q8, rq8, valid, sy == 0, a.write_cos != 0, pr);
+    if (valid) {
+        float o[16];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            o[2 * k] = yd[2 * k] + v[k];
+            o[2 * k + 1] = yd[2 * k + 1] + v[k];
+        }
+        float *dst = a.xnew[c] + p;
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++)
+            *reinterpret_cast<float4 *>(dst + 4 * q4) = make_float4(o[4 * q4], o[4 * q4 + 1], o[4 * q4 + 2], o[4 * q4 + 3]);
+        if (sy == 0) lane_store_prob(ch, by, bx, j, pr);
+    }
+}
+
+// Blocks per wave: 8 for full-resolution channels, 4 for 4:2:0 chroma.
+template <int MODE>
+constexpr int lane_blocks_per_wave() { return MODE == PM_SUB22 ? 4 : 8; }
+
+// One wave per unit = 8 (full resolution) or 4 (4:2:0 chroma) consecutive blocks of a block row;
+// units of channel 0 first.  u0, u1, u2: cumulative unit counts after channels 0, 1, 2.
 template <int C, int M0, int M1, int M2>
 __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t u0, uint32_t u1, uint32_t u2)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
     if (unit >= u2) return;
-    const int g = lane >> 3, j = lane & 7;
-    auto run = [&](auto MODE_C, int c, uint32_t local) {
-        constexpr int MODE = decltype(MODE_C)::value;
+    auto run = [&](auto MODE_C, auto C_C, uint32_t local) {
+        constexpr int MODE = decltype(MODE_C)::value, c = decltype(C_C)::value;
+        constexpr int BPW = lane_blocks_per_wave<MODE>();
         const ChanDev &ch = a.ch[c];
-        const uint32_t per_row = (ch.bw + 7) / 8;
-        const int by = (int)(local / per_row), bx = (int)(local % per_row) * 8 + g;
-        lane_unit<MODE>(a, c, by, bx, j, bx < (int)ch.bw);
+        const uint32_t per_row = (ch.bw + BPW - 1) / BPW;
+        const int by = (int)(local / per_row), bx = (int)(local % per_row) * BPW + lane % BPW;
+        if constexpr (MODE == PM_SUB22) lane_unit_sub22(a, c, by, bx, lane / BPW, bx < (int)ch.bw);
+        else lane_unit_full(a, c, by, bx, lane / BPW, bx < (int)ch.bw);
     };
-    if (unit < u0) run(IC<M0>{}, 0, unit);
-    else if (C > 1 && unit < u1) run(IC<M1>{}, 1, unit - u0);
-    else if (C > 2) run(IC<M2>{}, 2, unit - u1);
+    if (unit < u0) run(IC<M0>{}, IC<0>{}, unit);
+    else if (C > 1 && unit < u1) run(IC<M1>{}, IC<1>{}, unit - u0);
+    else if (C > 2) run(IC<M2>{}, IC<2>{}, unit - u1);
 }
 
 }  // namespace
@@ -1708,7 +1755,7 @@ struct artefact_hip_plan {
     int layout = 0;  // LAYOUT_*: which k_project variant runs
     int layout_fused = 0;  // the fused layout this plan's channels allow (LAYOUT_GENERIC if none)
     int lane_project = 0;  // fused layouts: 1 = k_project_lane (no LDS), 0 = k_project_fused
-    uint32_t lane_units[3] = {0, 0, 0};  // k_project_lane: cumulative units (8-block row pieces) per channel
+    uint32_t lane_units[3] = {0, 0, 0};  // k_project_lane: cumulative units (one wave's blocks) per channel
     uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
     float qmean[3] = {0, 0, 0};                   // mean quantisation step per channel (warm start)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
@@ -2202,13 +2249,16 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     {
         uint32_t acc = 0;
         for (uint32_t c = 0; c < 3; c++) {
-            if (c < pl->C) acc += ((pl->ch[c].bw + 7) / 8) * pl->ch[c].bh;
+            if (c < pl->C) {
+                const uint32_t bpw = pl->ch[c].mode == PM_SUB22 ? 4 : 8;  // blocks per wave, k_project_lane
+                acc += ((pl->ch[c].bw + bpw - 1) / bpw) * pl->ch[c].bh;
+            }
             pl->lane_units[c] = acc;
         }
     }
     // Untuned default, from measurement on MI355X: the LDS-free kernel for full-resolution
     // layouts (4096x3072 4:4:4: 144 vs 194 us), the LDS kernel for 4:2:0 (2048x1536: 36 vs 41 us).
-    pl->lane_project = pl->layout == LAYOUT_420 ? 0 : 1;
+    pl->lane_project = 1;  // measured faster than k_project_fused on every fused layout
     if (const char *e = getenv("ARTEFACT_PROJECT_KERNEL")) pl->lane_project = strcmp(e, "lds") != 0;
     if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
         if (atoi(e)) pl->layout = LAYOUT_GENERIC;
