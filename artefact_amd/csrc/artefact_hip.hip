@@ -1381,14 +1381,18 @@ __global__ __launch_bounds__(NT) void k_colour(ColourArgs a)
 // channels all cover the grid at full resolution (4:4:4, luma only).  Not bit-identical to
 // the reference: each product keeps about 16 significant bits.
 //
-// A wave owns a 16x16 super-tile (2x2 blocks) and holds it in the MFMA accumulator layout:
-// lane l, item t is element (row 4*(l/16)+t, column l%16).  With Db = blockdiag(D, D) (D the
-// orthonormal 8-point DCT-II matrix), one stage computes C = A * B with B a constant and A
-// the TRANSPOSE of the previous stage's result, because the A operand layout (lane l holds
-// A[l%16][4*(l/16)+t]) is the accumulator layout transposed.  So two chained stages give
-//   forward:  Y = (X^T Db^T)^T Db^T = Db X Db^T,    inverse:  X = (Y^T Db)^T Db = Db^T Y Db
-// per 8x8 block, with no shuffles, no LDS and no barriers.  Each stage is three MFMAs:
-// a_lo*b_hi + a_hi*b_lo + a_hi*b_hi (the a_lo*b_lo term, ~2^-16 relative, is dropped).
+// A wave owns a 16x16 super-tile (2x2 blocks).  The image enters as the A operand: lane l,
+// item t holds X[l%16][4*(l/16)+t], four consecutive pixels of one row, so loads and stores
+// are one float4 per lane and plane.  With Db = blockdiag(D, D) (D the orthonormal 8-point
+// DCT-II matrix) one stage computes C = A * B with a constant B.  The accumulator layout
+// (lane l, item t holds C[4*(l/16)+t][l%16]) read as the next stage's A operand is C^T, so
+// two chained stages give
+//   forward:  (X Db^T)^T Db^T = Db X^T Db^T = Y^T      with Y = Db X Db^T,
+//   inverse:  (Y Db)^T Db = Db^T Y^T Db = X'^T         with X' = Db^T Y Db,
+// and the accumulator then holds Y (resp. X') in the A operand layout again: lane l has
+// coefficients (u = l%16 mod 8, v = 4*(l/16) mod 8 + t) and, after the inverse, pixels of the
+// same row and columns it loaded.  No shuffles, no LDS, no barriers.  Each stage is three
+// MFMAs: a_lo*b_hi + a_hi*b_lo + a_hi*b_hi (the a_lo*b_lo term, ~2^-16 relative, is dropped).
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -1402,14 +1406,25 @@ __device__ __forceinline__ float bf16_to_f32(short h) { return __uint_as_float((
 struct SplitB {
     bf16x4 hi, lo;
 };
+typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+// Two f32 to bf16 with round-to-nearest-even (v_cvt_pk_bf16_f32); the same bits as bf16_rne
+// for finite inputs.  Element 0 in the low half.
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float x0, float x1)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){x0, x1}, bf16x2_hw));
+}
+
 __device__ __forceinline__ f32x4 mfma_stage(const f32x4 a, const SplitB &b)
 {
-    bf16x4 ah, al;
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        ah[t] = bf16_rne(a[t]);
-        al[t] = bf16_rne(a[t] - bf16_to_f32(ah[t]));
-    }
+    // a = hi + lo with hi = bf16(a) and lo = bf16(a - hi) (a - hi is exact in f32).
+    const uint32_t h01 = cvt_pk_bf16(a[0], a[1]), h23 = cvt_pk_bf16(a[2], a[3]);
+    const uint32_t l01 = cvt_pk_bf16(a[0] - __uint_as_float(h01 << 16), a[1] - __uint_as_float(h01 & 0xFFFF0000u));
+    const uint32_t l23 = cvt_pk_bf16(a[2] - __uint_as_float(h23 << 16), a[3] - __uint_as_float(h23 & 0xFFFF0000u));
+    const bf16x4 ah = __builtin_bit_cast(bf16x4, (u32x2){h01, h23});
+    const bf16x4 al = __builtin_bit_cast(bf16x4, (u32x2){l01, l23});
     f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
     c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, b.hi, c, 0, 0, 0);
     c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, b.lo, c, 0, 0, 0);
@@ -1417,82 +1432,111 @@ __device__ __forceinline__ f32x4 mfma_stage(const f32x4 a, const SplitB &b)
     return c;
 }
 
+// Orthonormal 8-point DCT-II basis D[u][x] = c(u) cos((2x + 1) u pi / 16), c(0) = 1/sqrt(8),
+// c(u > 0) = 1/2, rounded to f32 (the B operands of the precision form).
+__constant__ float kDct8[64] = {
+    0x1.6a09e6p-2f, 0x1.6a09e6p-2f, 0x1.6a09e6p-2f, 0x1.6a09e6p-2f, 0x1.6a09e6p-2f, 0x1.6a09e6p-2f, 0x1.6a09e6p-2f, 0x1.6a09e6p-2f,
+    0x1.f6297cp-2f, 0x1.a9b662p-2f, 0x1.1c73b4p-2f, 0x1.8f8b84p-4f, -0x1.8f8b84p-4f, -0x1.1c73b4p-2f, -0x1.a9b662p-2f, -0x1.f6297cp-2f,
+    0x1.d906bcp-2f, 0x1.87de2ap-3f, -0x1.87de2ap-3f, -0x1.d906bcp-2f, -0x1.d906bcp-2f, -0x1.87de2ap-3f, 0x1.87de2ap-3f, 0x1.d906bcp-2f,
+    0x1.a9b662p-2f, -0x1.8f8b84p-4f, -0x1.f6297cp-2f, -0x1.1c73b4p-2f, 0x1.1c73b4p-2f, 0x1.f6297cp-2f, 0x1.8f8b84p-4f, -0x1.a9b662p-2f,
+    0x1.6a09e6p-2f, -0x1.6a09e6p-2f, -0x1.6a09e6p-2f, 0x1.6a09e6p-2f, 0x1.6a09e6p-2f, -0x1.6a09e6p-2f, -0x1.6a09e6p-2f, 0x1.6a09e6p-2f,
+    0x1.1c73b4p-2f, -0x1.f6297cp-2f, 0x1.8f8b84p-4f, 0x1.a9b662p-2f, -0x1.a9b662p-2f, -0x1.8f8b84p-4f, 0x1.f6297cp-2f, -0x1.1c73b4p-2f,
+    0x1.87de2ap-3f, -0x1.d906bcp-2f, 0x1.d906bcp-2f, -0x1.87de2ap-3f, -0x1.87de2ap-3f, 0x1.d906bcp-2f, -0x1.d906bcp-2f, 0x1.87de2ap-3f,
+    0x1.8f8b84p-4f, -0x1.1c73b4p-2f, 0x1.a9b662p-2f, -0x1.f6297cp-2f, 0x1.f6297cp-2f, -0x1.a9b662p-2f, 0x1.1c73b4p-2f, -0x1.8f8b84p-4f,
+};
+
 template <int C>
 __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = a.W, H = a.H;
-    const int x0 = blockIdx.x * TW + wave * 16, y0 = blockIdx.y * TH;
-    const int col = lane & 15, rq = (lane >> 4) * 4;  // element (rq + t, col) of the super-tile
-    // B operands: element (k = rq + t, n = col).  Forward stages use Db^T, inverse stages Db.
+    // The wave's four 8x8 blocks sit side by side in one 8-row band: super-tile blocks (0,0),
+    // (0,1), (1,0), (1,1) are image blocks 0, 1, 2, 3 from the left; a workgroup covers 2 x 2
+    // such bands of 32 columns (its TW x TH tile).
+    const int x0 = blockIdx.x * TW + (wave & 1) * 32, y0 = blockIdx.y * TH + (wave >> 1) * 8;
+    // X enters in the A operand layout (lane l: row l % 16, columns 4 (l / 16) .. + 3), so each
+    // lane loads and stores one float4 per plane.  B operands: lane l holds element
+    // (k = 4 (l / 16) + t, n = l % 16).  Forward stages use Db^T, inverse stages Db.
+    const int r = lane & 15, cq = (lane >> 4) * 4;
     SplitB fwd, inv;
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        const int k = rq + t, n = col;
+        const int k = cq + t, n = r;
         const bool same = (k >> 3) == (n >> 3);
         const int ku = k & 7, nu = n & 7;
-        auto dct = [](int u, int x) -> float {
-            const double cu = u == 0 ? 0.35355339059327376220 : 0.5;
-            return (float)(cu * cos((2 * x + 1) * u * 3.14159265358979323846 / 16.0));
-        };
-        const float vf = same ? dct(nu, ku) : 0.0f;  // Db^T[k][n] = D[n][k]
-        const float vi = same ? dct(ku, nu) : 0.0f;  // Db[k][n] = D[k][n]
+        const float vf = same ? kDct8[nu * 8 + ku] : 0.0f;  // Db^T[k][n] = D[n][k]
+        const float vi = same ? kDct8[ku * 8 + nu] : 0.0f;  // Db[k][n] = D[k][n]
         fwd.hi[t] = bf16_rne(vf);
         fwd.lo[t] = bf16_rne(vf - bf16_to_f32(fwd.hi[t]));
         inv.hi[t] = bf16_rne(vi);
         inv.lo[t] = bf16_rne(vi - bf16_to_f32(inv.hi[t]));
     }
-    const int px = x0 + col;
-    const bool xin = px < W;
+    // This lane's four pixels: super-tile row r, columns cq .. cq + 3, i.e. image row
+    // y0 + r % 8, columns x0 + 16 (r / 8) + cq .. + 3 (W and H are multiples of 8, so the four are
+    // all inside the grid or all outside).  Every load of every channel is issued here, without
+    // branches: a lane outside reads pixel 0 and coefficient 0 and stores nothing.
+    const int py = y0 + (r & 7), px = x0 + 16 * (r >> 3) + cq;
+    const bool in = px < W && py < H;
+    const size_t pix = in ? (size_t)py * W + px : 0;
+    const int kq = (r & 7) * 8 + (cq & 7);  // natural-order index of the first coefficient
+    auto coef_index = [&](int c) -> uint32_t {  // block order, < 2^31 by the plan's size limit
+        return in ? ((uint32_t)(py >> 3) * a.ch[c].bw + (uint32_t)(px >> 3)) * 64u + (uint32_t)kq : 0u;
+    };
     const float step = a.step;
+    float4 yl[C], gl[C];
+    int2 cl[C];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+        yl[c] = *reinterpret_cast<const float4 *>(a.y[c] + pix);
+        gl[c] = *reinterpret_cast<const float4 *>(a.g[c] + pix);
+        cl[c] = *reinterpret_cast<const int2 *>(a.ch[c].coef + coef_index(c));
+    }
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const ChanDev &ch = a.ch[c];
         const float norm = a.norms[c];
         const double rn = a.rnorm[c];
+        const float ys[4] = {yl[c].x, yl[c].y, yl[c].z, yl[c].w};
+        const float gs[4] = {gl[c].x, gl[c].y, gl[c].z, gl[c].w};
         f32x4 xv;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const int py = y0 + rq + t;
-            float v = 0.0f;
-            if (xin && py < H) {
-                const size_t p = (size_t)py * W + px;
-                v = a.y[c][p];
-                // Descent step on the extrapolated image (compute_step.rs:61-65).
-                if (norm != 0.0f) v -= step * qdiv(a.g[c][p], rn);
-            }
-            xv[t] = v;
+            float v = ys[t];
+            // Descent step on the extrapolated image (compute_step.rs:61-65).
+            if (norm != 0.0f) v -= step * qdiv(gs[t], rn);
+            xv[t] = in ? v : 0.0f;
         }
-        // Forward DCT, clamp to [(c - 1/2) q, (c + 1/2) q], prob difference (cos - c q) / q^2.
+        // Forward DCT (row pass, then column pass): this lane then holds coefficients
+        // (u = r & 7, v = (cq & 7) + t) of its block.  Clamp to [(c - 1/2) q, (c + 1/2) q] and
+        // form the prob difference (cos - c q) / q^2.
         const f32x4 yv = mfma_stage(mfma_stage(xv, fwd), fwd);
+        const float4 q4 = *reinterpret_cast<const float4 *>(ch.quant + kq);
+        const float qs[4] = {q4.x, q4.y, q4.z, q4.w};
+        const int cw[2] = {cl[c].x, cl[c].y};
         f32x4 ov, dv;
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const int py = y0 + rq + t;
-            const int k = ((rq + t) & 7) * 8 + (col & 7);
-            float o = 0.0f, d = 0.0f;
-            if (xin && py < H) {
-                const size_t base = ((size_t)(py >> 3) * ch.bw + (px >> 3)) * 64;
-                const float q = ch.quant[k];
-                const float cf = (float)ch.coef[base + k];
-                o = clamp_ref(yv[t], (cf - 0.5f) * q, (cf + 0.5f) * q);
-                d = qdiv(o - cf * q, ch.rqq[k]);
-                if (a.write_cos) ch.cos[base + k] = o;
-            }
-            ov[t] = o;
-            dv[t] = d;
+            const float q = qs[t];
+            const float cf = (float)(short)((uint32_t)cw[t >> 1] >> (16 * (t & 1)));
+            float o = clamp_ref(yv[t], (cf - 0.5f) * q, (cf + 0.5f) * q);
+            float d = qdiv(o - cf * q, ch.rqq[kq + t]);
+            ov[t] = in ? o : 0.0f;
+            dv[t] = in ? d : 0.0f;
         }
+        if (in && a.write_cos)
+            *reinterpret_cast<float4 *>(ch.cos + coef_index(c)) = make_float4(ov[0], ov[1], ov[2], ov[3]);
+        // Inverse DCT (row pass, then column pass): this lane then holds row py, columns px ..
+        // px + 3 of the projected image and of the prob plane.
         const f32x4 xn = mfma_stage(mfma_stage(ov, inv), inv);
         const bool prob_on = ch.alpha_p != 0.0f;
         f32x4 pn = {0.0f, 0.0f, 0.0f, 0.0f};
         if (prob_on) pn = mfma_stage(mfma_stage(dv, inv), inv);
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int py = y0 + rq + t;
-            if (xin && py < H) {
-                const size_t p = (size_t)py * W + px;
-                a.xnew[c][p] = xn[t];
-                if (prob_on) ch.prob[(size_t)py * ch.rw + px] = ch.alpha_p * pn[t];
+        if (in) {
+            *reinterpret_cast<float4 *>(a.xnew[c] + pix) = make_float4(xn[0], xn[1], xn[2], xn[3]);
+            if (prob_on) {
+                const float ap = ch.alpha_p;
+                *reinterpret_cast<float4 *>(ch.prob + (size_t)py * ch.rw + px) =
+                    make_float4(ap * pn[0], ap * pn[1], ap * pn[2], ap * pn[3]);
             }
         }
     }
