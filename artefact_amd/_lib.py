@@ -104,6 +104,9 @@ SIGNATURES = [
     ("artefact_hip_plan_last_norms", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
     ("artefact_hip_plan_first_gradient", ctypes.c_int, [ctypes.c_void_p, FLOAT_PP]),
     ("artefact_hip_plan_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    ("artefact_hip_plan_set_timing", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("artefact_hip_plan_timing", ctypes.c_int,
+     [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint32)]),
     ("artefact_hip_plan_objective", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     ("artefact_hip_plan_stream", ctypes.c_void_p, [ctypes.c_void_p]),
     ("artefact_hip_plan_set_graph", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),

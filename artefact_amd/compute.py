@@ -176,6 +176,18 @@ class Plan:
                    "artefact_hip_plan_profile")
         return {"k_gradient_ms": float(out[0]), "k_project_ms": float(out[1]), "loop_ms": float(out[2])}
 
+    def set_timing(self, launches: int) -> None:
+        """Record HIP events around the next ``launches`` iterate() launches (0: off)."""
+        _lib.check(self._lib.artefact_hip_plan_set_timing(self._h, int(launches)), "artefact_hip_plan_set_timing")
+
+    def timing(self) -> tuple[float, int]:
+        """(summed duration in ms, number of timed launches) of the iterate() launches."""
+        ms = ctypes.c_float(0.0)
+        n = ctypes.c_uint32(0)
+        _lib.check(self._lib.artefact_hip_plan_timing(self._h, ctypes.byref(ms), ctypes.byref(n)),
+                   "artefact_hip_plan_timing")
+        return float(ms.value), int(n.value)
+
     def objective(self) -> tuple[float, np.ndarray]:
         """f64 objective of the current iterate: (total, [TV, second-order, prob])."""
         t = np.zeros(3, dtype=np.float64)

@@ -1823,6 +1823,9 @@ struct artefact_hip_plan {
     bool initialized = false;
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;
+    // Optional event pairs around iterate()'s launches (artefact_hip_plan_set_timing).
+    std::vector<hipEvent_t> timing_events;
+    uint32_t timing_used = 0;  // pairs recorded
 };
 
 namespace {
@@ -1860,6 +1863,7 @@ void free_plan(artefact_hip_plan *pl)
     if (!pl) return;
     DeviceGuard guard(pl->device);
     if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
+    for (hipEvent_t e : pl->timing_events) (void)hipEventDestroy(e);
     void *bufs[] = {pl->buf[0], pl->buf[1], pl->ybuf, pl->g,        pl->cos,     pl->prob,  pl->coef, pl->quant,
                     pl->rqq,    pl->partials, pl->norms, pl->rnorm, pl->norm_sq};
     for (void *b : bufs)
@@ -2371,10 +2375,20 @@ int artefact_hip_plan_iterate(artefact_hip_plan *pl)
                 return ARTEFACT_E_HIP;
             }
         }
+    }
+    const bool timed = pl->timing_used < pl->timing_events.size() / 2;
+    if (timed && hipEventRecord(pl->timing_events[2 * pl->timing_used], pl->stream) != hipSuccess)
+        return ARTEFACT_E_HIP;
+    if (pl->use_graph) {
         if (hipGraphLaunch(pl->graph, pl->stream) != hipSuccess) return ARTEFACT_E_HIP;
     } else {
         rc = launch_iterations(pl, pl->stream);
         if (rc) return rc;
+    }
+    if (timed) {
+        if (hipEventRecord(pl->timing_events[2 * pl->timing_used + 1], pl->stream) != hipSuccess)
+            return ARTEFACT_E_HIP;
+        pl->timing_used++;
     }
     if (pl->iterations % 2) pl->cur = 1 - pl->cur;
     // The loop consumed the start image; a new solve needs a new init (the reference
@@ -2476,6 +2490,39 @@ int artefact_hip_plan_profile(artefact_hip_plan *pl, float *avg_ms)
     avg_ms[2] = total;
     pl->cur = cur;
     pl->initialized = false;
+    return ARTEFACT_OK;
+}
+
+int artefact_hip_plan_set_timing(artefact_hip_plan *pl, uint32_t launches)
+{
+    if (!pl) return ARTEFACT_E_ARGUMENT;
+    DeviceGuard guard(pl->device);
+    if (pl->stream && hipStreamSynchronize(pl->stream) != hipSuccess) return ARTEFACT_E_HIP;
+    for (hipEvent_t e : pl->timing_events) (void)hipEventDestroy(e);
+    pl->timing_events.clear();
+    pl->timing_used = 0;
+    for (uint32_t i = 0; i < 2 * launches; i++) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return ARTEFACT_E_HIP;
+        pl->timing_events.push_back(e);
+    }
+    return ARTEFACT_OK;
+}
+
+int artefact_hip_plan_timing(artefact_hip_plan *pl, float *total_ms, uint32_t *launches)
+{
+    if (!pl || !total_ms || !launches) return ARTEFACT_E_ARGUMENT;
+    DeviceGuard guard(pl->device);
+    double total = 0.0;
+    for (uint32_t i = 0; i < pl->timing_used; i++) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(pl->timing_events[2 * i + 1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, pl->timing_events[2 * i], pl->timing_events[2 * i + 1]) != hipSuccess)
+            return ARTEFACT_E_HIP;
+        total += ms;
+    }
+    *total_ms = (float)total;
+    *launches = pl->timing_used;
     return ARTEFACT_OK;
 }
 

@@ -105,6 +105,13 @@ int artefact_hip_plan_first_gradient(artefact_hip_plan *plan, float *const *out_
  * iterate, it consumes the initialised state.  No reference counterpart: the
  * reference's only timing hook is its benchmark exit (lib.rs:155-157). */
 int artefact_hip_plan_profile(artefact_hip_plan *plan, float *avg_ms);
+/* Timing of the product path itself: after set_timing(plan, n), the next n calls of
+ * artefact_hip_plan_iterate record a HIP event pair on the plan's stream around their
+ * launch (the captured graph, or the eager loop), without synchronising.  timing() waits
+ * for the last pair and returns the summed duration and the number of timed launches.
+ * set_timing(plan, 0) switches it off.  No reference counterpart (see profile above). */
+int artefact_hip_plan_set_timing(artefact_hip_plan *plan, uint32_t launches);
+int artefact_hip_plan_timing(artefact_hip_plan *plan, float *total_ms, uint32_t *launches);
 /* Objective of the current iterate in f64: terms[0] = sum alpha*n(p) (TV form of
  * compute_step_tv.rs:62-71), terms[1] = sum a2*n2(p) (second-order form of
  * compute_step_tv2.rs:53-90), terms[2] = sum_c alpha_p/2 * sum ((DCT(down x)_j - c_j q_j)/q_j)^2.
