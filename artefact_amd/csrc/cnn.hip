@@ -70,9 +70,16 @@ constexpr ParamLayout PL = param_layout();
 // MODE_FINAL: + bias, channel 0 only, f32 out = residual feature + conv (the prediction).
 enum { MODE_RELU = 0, MODE_LINEAR = 1, MODE_FINAL = 2 };
 
-// Pixels of one row per workgroup: 4 waves x 16 pixels, NSEG times (the weights stay in
-// registers across the segments).
-constexpr int NSEG = 8;
+// Output tile of one workgroup: 64 pixels (wave w: pixels 16w .. 16w+15) x RT rows.  The input
+// tile (RT + 2 rows x 66 pixels x CI channels, zero outside the plane) is staged in LDS once;
+// each wave walks its input rows and feeds every B operand it reads to the (up to three)
+// output rows that use it.  Every output still accumulates its taps in the order (dy, dx, kc),
+// so the result does not depend on the tiling.
+constexpr int RT = 8;
+constexpr int TILE_PX = 66;
+
+template <int CI>
+constexpr int lds_px_bytes() { return CI * 2 + 8; }  // padded pixel stride (bank spread), a multiple of 8
 
 template <int CI, int CO, int MODE>
 __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, const bf16x4 *__restrict__ wpk,
@@ -80,11 +87,10 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
                                              float *__restrict__ pred, const float *__restrict__ resid, int H,
                                              int W)
 {
-    constexpr int KC = CI / 16, MH = CO / 16;
+    constexpr int KC = CI / 16, MH = CO / 16, PXB = lds_px_bytes<CI>();
+    __shared__ __attribute__((aligned(16))) uint8_t tile[(RT + 2) * TILE_PX * PXB];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b = blockIdx.z, y = blockIdx.y;
-    const int xb = blockIdx.x * 64 * NSEG + wave * 16;
-    if (xb >= W) return;
+    const int b = blockIdx.z, y0 = blockIdx.y * RT, x0 = blockIdx.x * 64;
     bf16x4 wr[9][KC][MH];
 #pragma unroll
     for (int t = 0; t < 9; t++)
@@ -92,37 +98,58 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
         for (int kc = 0; kc < KC; kc++)
 #pragma unroll
             for (int mh = 0; mh < MH; mh++) wr[t][kc][mh] = wpk[((t * KC + kc) * MH + mh) * 64 + lane];
+    // Stage the input tile: 16-byte units (8 channels), consecutive threads on consecutive units.
+    constexpr int UPX = CI / 8, UNITS = (RT + 2) * TILE_PX * UPX;
+    for (int u = threadIdx.x; u < UNITS; u += CT) {
+        const int part = u % UPX, px = (u / UPX) % TILE_PX, r = u / (UPX * TILE_PX);
+        const int gy = y0 - 1 + r, gx = x0 - 1 + px;
+        const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const size_t g = (((size_t)b * H + (ok ? gy : 0)) * W + (ok ? gx : 0)) * CI + part * 8;
+        const uint4 v = *reinterpret_cast<const uint4 *>(in + g);
+        // The padded pixel stride is a multiple of 8 bytes, not 16: two 8-byte stores.
+        uint2 *dst = reinterpret_cast<uint2 *>(tile + (r * TILE_PX + px) * PXB + part * 16);
+        dst[0] = ok ? make_uint2(v.x, v.y) : make_uint2(0, 0);
+        dst[1] = ok ? make_uint2(v.z, v.w) : make_uint2(0, 0);
+    }
+    __syncthreads();
     const int n = lane & 15, kq = (lane >> 4) * 4;
-    for (int seg = 0; seg < NSEG; seg++) {
-        const int x0 = xb + seg * 64;
-        if (x0 >= W) break;
-        f32x4 acc[MH];
+    f32x4 acc[RT][MH];
 #pragma unroll
-        for (int mh = 0; mh < MH; mh++) acc[mh] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ro = 0; ro < RT; ro++)
 #pragma unroll
-        for (int dy = 0; dy < 3; dy++) {
-            const int yy = y + dy - 1;
+        for (int mh = 0; mh < MH; mh++) acc[ro][mh] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int dx = 0; dx < 3; dx++) {
-                const int px = x0 + n + dx - 1;
-                const bool ok = yy >= 0 && yy < H && px >= 0 && px < W;
-                const size_t base = (((size_t)b * H + (ok ? yy : 0)) * W + (ok ? px : 0)) * CI + kq;
+    for (int ir = 0; ir < RT + 2; ir++) {
+        bf16x4 bv[3][KC];
 #pragma unroll
-                for (int kc = 0; kc < KC; kc++) {
-                    bf16x4 bv = {0, 0, 0, 0};
-                    if (ok) bv = *reinterpret_cast<const bf16x4 *>(in + base + kc * 16);
+        for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++)
+                bv[dx][kc] = *reinterpret_cast<const bf16x4 *>(tile + (ir * TILE_PX + wave * 16 + n + dx) * PXB +
+                                                               (kc * 16 + kq) * 2);
+#pragma unroll
+        for (int dx = 0; dx < 3; dx++)
+#pragma unroll
+            for (int kc = 0; kc < KC; kc++)
+#pragma unroll
+                for (int dy = 2; dy >= 0; dy--) {  // output row ro = ir - dy uses this input row as tap dy
+                    const int ro = ir - dy;
+                    if (ro < 0 || ro >= RT) continue;
 #pragma unroll
                     for (int mh = 0; mh < MH; mh++)
-                        acc[mh] =
-                            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[dy * 3 + dx][kc][mh], bv, acc[mh], 0, 0, 0);
+                        acc[ro][mh] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wr[dy * 3 + dx][kc][mh], bv[dx][kc],
+                                                                                acc[ro][mh], 0, 0, 0);
                 }
-            }
-        }
-        const int px = x0 + n;
-        if (px >= W) continue;
+    }
+    const int px = x0 + wave * 16 + n;
+    if (px >= W) return;
+#pragma unroll
+    for (int ro = 0; ro < RT; ro++) {
+        const int y = y0 + ro;
+        if (y >= H) break;
         const size_t p = ((size_t)b * H + y) * W + px;
         if (MODE == MODE_FINAL) {
-            if (kq == 0) pred[p] = resid[p] + (acc[0][0] + bias[0]);
+            if (kq == 0) pred[p] = resid[p] + (acc[ro][0][0] + bias[0]);
             continue;
         }
 #pragma unroll
@@ -130,7 +157,7 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
             uint16_t v[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                float r = acc[mh][t];
+                float r = acc[ro][mh][t];
                 if (MODE == MODE_RELU) {
                     const int co = mh * 16 + kq + t;
                     r += bias[co];
@@ -399,7 +426,7 @@ template <int CI, int CO, int MODE>
 void conv(artefact_cnn *net, const uint16_t *in, const bf16x4 *w, const float *bias, uint16_t *out, float *pred,
           const float *resid, int B, int H, int W, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_conv<CI, CO, MODE>), dim3((W + 64 * NSEG - 1) / (64 * NSEG), H, B), dim3(CT), 0, s, in, w,
+    hipLaunchKernelGGL((k_conv<CI, CO, MODE>), dim3((W + 63) / 64, (H + RT - 1) / RT, B), dim3(CT), 0, s, in, w,
                        bias, out, pred, resid, H, W);
 }
 
@@ -481,11 +508,11 @@ int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const floa
     if (!net || !planes || !qsteps || !out || !nplanes || nplanes > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
     const size_t plane = (size_t)h * w;
     if (!ensure_workspace(net, plane)) return ARTEFACT_E_HIP;
-    bool ok = true;
+    // One upload of every plane's step (net->qstep holds 64 floats).
+    bool ok = hipMemcpyAsync(net->qstep, qsteps, (size_t)nplanes * 4, hipMemcpyHostToDevice, s) == hipSuccess;
     for (uint32_t i = 0; i < nplanes && ok; i++) {
-        ok = hipMemcpyAsync(net->qstep, qsteps + i, 4, hipMemcpyHostToDevice, s) == hipSuccess;
-        hipLaunchKernelGGL(k_pack_input, dim3(blocks(plane)), dim3(CT), 0, s, planes[i], (const float *)net->qstep,
-                           plane, 1, net->act[0], net->resid);
+        hipLaunchKernelGGL(k_pack_input, dim3(blocks(plane)), dim3(CT), 0, s, planes[i],
+                           (const float *)(net->qstep + i), plane, 1, net->act[0], net->resid);
         forward(net, 1, (int)h, (int)w, s);
         hipLaunchKernelGGL(k_unpack_pred, dim3(blocks(plane)), dim3(CT), 0, s, (const float *)net->pred, plane,
                            out[i]);
