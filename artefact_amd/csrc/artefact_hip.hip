@@ -191,17 +191,16 @@ __device__ __forceinline__ float clamp_ref(float v, float lo, float hi)
     return v;
 }
 
-// Reciprocal of a positive finite f32 in f64, accurate to ~2^-52: hardware estimate
-// (v_rcp_f64) refined by two Newton steps.
+// Reciprocal of a positive finite f32 in f64, accurate to ~2^-52: the hardware estimate r
+// (v_rcp_f64, relative error e = 1 - b r of order 2^-23) refined by one third-order step
+// r (1 + e + e^2), whose truncation error e^3 (~2^-69) is far below the final rounding.
 __device__ __forceinline__ double recip64(float b)
 {
     const double d = (double)b;
-    double r = __builtin_amdgcn_rcp(d);
-    double e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-d, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    return r;
+    const double r = __builtin_amdgcn_rcp(d);
+    const double e = __builtin_fma(-d, r, 1.0);
+    const double t = __builtin_fma(e, e, e);
+    return __builtin_fma(r, t, r);
 }
 
 // a / b for f32 a and b, given r ~ 1/b (see the header comment): bit-identical to IEEE.
@@ -440,7 +439,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                 // A zero norm or a source outside the image contributes nothing: its
                 // reciprocal is set to 0, so every contribution becomes +-0, which adds
                 // nothing to an accumulator that is never -0.  (An infinite norm would give
-                // recip64 = NaN where IEEE division gives a / inf = 0; it needs squared
+                // a NaN reciprocal where IEEE division gives a / inf = 0; it needs squared
                 // differences above 3.4e38, which int16 coefficients and u16 tables cannot
                 // produce.)
                 const float n = sqrtf(nacc);
@@ -460,8 +459,11 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                     kc.tvd = qdiv(atv * gy, rn);
                     if (TGV) {
                         const float xx = gxx[c], yy = gyy[c], sm = sym[c];
-                        kc.tgs = atg * qdiv(-(2.0f * yy + (2.0f * xx + 2.0f * sm)), rn2);
-                        kc.tgx = atg * qdiv(sm + xx, rn2);
+                        const float sx = sm + xx;
+                        // -(2 yy + (2 xx + 2 sm)) == -2 (yy + (sm + xx)) bit for bit: doubling
+                        // is exact and commutes with rounding (a subnormal sum is exact).
+                        kc.tgs = atg * qdiv(-2.0f * (yy + sx), rn2);
+                        kc.tgx = atg * qdiv(sx, rn2);
                         kc.tgy = atg * qdiv(yy + sm, rn2);
                         kc.tgd = atg * qdiv(-sm, rn2);
                     }
@@ -2014,6 +2016,27 @@ const void *gradient_kernel(uint32_t C, bool tgv)
 // capacity runs a partial second round.  Pick the smallest RW whose grid (4 strips per
 // workgroup, stacked) fits the device's resident workgroups at once.  An override
 // (ARTEFACT_KG_ROWS) is honoured.
+// Resident k_gradient workgroups per CU and the CU count (4 and 256 if the queries fail).
+void gradient_residency(int device, uint32_t C, bool tgv, int &per_cu, int &cus)
+{
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gradient_kernel(C, tgv), NT, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1 ||
+        cus < 1) {
+        per_cu = 4;
+        cus = 256;
+    }
+}
+
+// Smallest strip height (>= 2: a 1-row strip spends 5 row loads per output row) whose grid has
+// at most `wgs` workgroups.
+int rows_for_workgroups(uint32_t W, uint32_t H, uint64_t wgs)
+{
+    const uint64_t cols = (W + KG_COLS - 1) / KG_COLS;
+    for (uint32_t r = 2; r < H; r++)
+        if (cols * ((H + WAVES * r - 1) / (WAVES * r)) <= wgs) return (int)r;
+    return (int)std::max(H, 1u);
+}
+
 int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv)
 {
     if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
@@ -2021,17 +2044,8 @@ int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv)
         if (r >= 1 && r <= 4096) return r;
     }
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gradient_kernel(C, tgv), NT, 0) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1 ||
-        cus < 1) {
-        per_cu = 4;
-        cus = 256;
-    }
-    const uint64_t cols = (W + KG_COLS - 1) / KG_COLS, resident = (uint64_t)per_cu * cus;
-    // At least 2 rows: a 1-row strip spends 5 row loads per output row.
-    for (uint32_t r = 2; r < H; r++)
-        if (cols * ((H + WAVES * r - 1) / (WAVES * r)) <= resident) return (int)r;
-    return (int)std::max(H, 1u);
+    gradient_residency(device, C, tgv, per_cu, cus);
+    return rows_for_workgroups(W, H, (uint64_t)per_cu * cus);
 }
 
 // Output stage on `stream` from device planes (y, cb, cr; cb/cr ignored unless colour) into
@@ -2120,9 +2134,21 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
     if (best.ms < 0.0f && pl->iterations > 0) {
         const int r0 = pl->rows;
         std::vector<int> rows{r0};
-        if (!getenv("ARTEFACT_KG_ROWS"))
-            for (int r : {r0 + 1, r0 + 2, r0 - 1, r0 - 2, 2 * r0})
+        if (!getenv("ARTEFACT_KG_ROWS")) {
+            // Around the one-round height, and the heights that give every CU exactly
+            // per_cu - 1 or per_cu - 2 workgroups: the sweep is VALU-bound per SIMD, so its time
+            // follows the most loaded CU's (workgroups x strip steps).
+            int per_cu = 0, cus = 0;
+            gradient_residency(pl->device, pl->C, pl->tgv, per_cu, cus);
+            std::vector<int> cand{r0 + 1, r0 + 2, r0 - 1, r0 - 2, 2 * r0};
+            for (int k = per_cu - 1; k >= std::max(1, per_cu - 2); k--) {
+                const int rk = rows_for_workgroups(pl->W, pl->H, (uint64_t)k * cus);
+                cand.push_back(rk);
+                cand.push_back(rk + 1);
+            }
+            for (int r : cand)
                 if (r >= 2 && r <= (int)pl->H && std::find(rows.begin(), rows.end(), r) == rows.end()) rows.push_back(r);
+        }
         // Projection candidates: (layout, lane kernel) -- the fused layout with the LDS-free and
         // the LDS kernel, and the per-channel kernel.
         std::vector<std::pair<int, int>> projs{{pl->layout, pl->lane_project}};
