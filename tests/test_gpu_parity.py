@@ -249,3 +249,25 @@ def test_both_fused_projection_kernels_bit_exact(kernel, name, iters, monkeypatc
     ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, norm_mode="f64")
     for c, (a, b) in enumerate(zip(gpu, ref)):
         assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
+
+
+def test_timed_launches_same_bits_and_counted():
+    """Event pairs around iterate() (set_timing) change nothing in the result; the pairs are
+    counted, and launches past the requested number are not timed."""
+    wl = synth.workload("tgv_420_96x64", seed=29)
+    W, H = wl["max_w"], wl["max_h"]
+    print("case timing tgv_420_96x64 iters=6", flush=True)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], 6, W, H, norm_mode="f64")
+    with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], 6, W, H, mode=wl["mode"], graph=True) as plan:
+        plan.set_timing(2)
+        for _ in range(3):
+            plan.init()
+            plan.iterate()
+            out = plan.download()
+            for c, (a, b) in enumerate(zip(out, ref)):
+                assert bits_equal(a, b), f"channel {c}"
+        ms, n = plan.timing()
+        assert n == 2
+        assert ms > 0.0
+        plan.set_timing(0)
+        assert plan.timing() == (0.0, 0)
