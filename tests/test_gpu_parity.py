@@ -271,3 +271,16 @@ def test_timed_launches_same_bits_and_counted():
         assert ms > 0.0
         plan.set_timing(0)
         assert plan.timing() == (0.0, 0)
+
+
+@pytest.mark.parametrize("name,iters", [("tgv_420_2048", 3), ("tgv_444_4096", 2), ("tv_luma_512", 6)])
+def test_bit_exact_at_configured_sizes(name, iters):
+    """The configured workloads at their real shapes (default strip height and projection
+    kernel for each), a few iterations: bit-identical to the oracle with the f64 norm."""
+    wl = synth.workload(name, seed=0)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case {name} {W}x{H} iters={iters}", flush=True)
+    gpu = run_gpu(wl, iters)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, norm_mode="f64")
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
