@@ -336,7 +336,7 @@ __global__ __launch_bounds__(CT) void k_loss(const float *pred, const float *tar
     red[threadIdx.x] = e2;
     __syncthreads();
     for (int s = CT / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
@@ -423,7 +423,7 @@ void repack(artefact_cnn *net)
 }
 
 template <int CI, int CO, int MODE>
-void conv(artefact_cnn *net, const uint16_t *in, const bf16x4 *w, const float *bias, uint16_t *out, float *pred,
+void conv(const uint16_t *in, const bf16x4 *w, const float *bias, uint16_t *out, float *pred,
           const float *resid, int B, int H, int W, hipStream_t s)
 {
     hipLaunchKernelGGL((k_conv<CI, CO, MODE>), dim3((W + 63) / 64, (H + RT - 1) / RT, B), dim3(CT), 0, s, in, w,
@@ -433,15 +433,15 @@ void conv(artefact_cnn *net, const uint16_t *in, const bf16x4 *w, const float *b
 // Forward pass on the features in act[0] / resid; the prediction (feature units) in pred.
 void forward(artefact_cnn *net, int B, int H, int W, hipStream_t s)
 {
-    conv<16, 32, MODE_RELU>(net, net->act[0], net->wf[0], net->params + PL.b[0], net->act[1], nullptr, nullptr, B, H,
+    conv<16, 32, MODE_RELU>(net->act[0], net->wf[0], net->params + PL.b[0], net->act[1], nullptr, nullptr, B, H,
                             W, s);
-    conv<32, 32, MODE_RELU>(net, net->act[1], net->wf[1], net->params + PL.b[1], net->act[2], nullptr, nullptr, B, H,
+    conv<32, 32, MODE_RELU>(net->act[1], net->wf[1], net->params + PL.b[1], net->act[2], nullptr, nullptr, B, H,
                             W, s);
-    conv<32, 32, MODE_RELU>(net, net->act[2], net->wf[2], net->params + PL.b[2], net->act[3], nullptr, nullptr, B, H,
+    conv<32, 32, MODE_RELU>(net->act[2], net->wf[2], net->params + PL.b[2], net->act[3], nullptr, nullptr, B, H,
                             W, s);
-    conv<32, 32, MODE_RELU>(net, net->act[3], net->wf[3], net->params + PL.b[3], net->act[4], nullptr, nullptr, B, H,
+    conv<32, 32, MODE_RELU>(net->act[3], net->wf[3], net->params + PL.b[3], net->act[4], nullptr, nullptr, B, H,
                             W, s);
-    conv<32, 16, MODE_FINAL>(net, net->act[4], net->wf[4], net->params + PL.b[4], nullptr, net->pred, net->resid, B,
+    conv<32, 16, MODE_FINAL>(net->act[4], net->wf[4], net->params + PL.b[4], nullptr, net->pred, net->resid, B,
                              H, W, s);
 }
 
@@ -463,14 +463,14 @@ bool backward(artefact_cnn *net, int B, int H, int W, double *loss)
                        (const float *)net->target, px, net->g16, net->lpart);
     // layer 4 (32 -> 16 padded): weights from act[4]; data gradient into g32a
     wgrad<32, 16>(net, 4, net->act[4], net->g16, B, H, W);
-    conv<16, 32, MODE_LINEAR>(net, net->g16, net->wb[4], nullptr, net->g32a, nullptr, nullptr, B, H, W,
+    conv<16, 32, MODE_LINEAR>(net->g16, net->wb[4], nullptr, net->g32a, nullptr, nullptr, B, H, W,
                               net->stream);
     hipLaunchKernelGGL(k_relu_bwd, dim3(blocks(px * 32)), dim3(CT), 0, net->stream, net->g32a,
                        (const uint16_t *)net->act[4], px * 32);
     uint16_t *g = net->g32a, *gn = net->g32b;
     for (int l = 3; l >= 1; l--) {
         wgrad<32, 32>(net, l, net->act[l], g, B, H, W);
-        conv<32, 32, MODE_LINEAR>(net, g, net->wb[l], nullptr, gn, nullptr, nullptr, B, H, W, net->stream);
+        conv<32, 32, MODE_LINEAR>(g, net->wb[l], nullptr, gn, nullptr, nullptr, B, H, W, net->stream);
         hipLaunchKernelGGL(k_relu_bwd, dim3(blocks(px * 32)), dim3(CT), 0, net->stream, gn,
                            (const uint16_t *)net->act[l], px * 32);
         std::swap(g, gn);
