@@ -279,11 +279,6 @@ struct IterArgs {
     uint32_t kg_rows;        // k_gradient output rows per wave
 };
 
-// Contributions of one source pixel: TV self/right/down, second-order self/x/y/diagonal.
-struct Contrib {
-    float tvs, tvr, tvd, tgs, tgx, tgy, tgd;
-};
-
 // Raw buffer resource over `bytes` bytes: loads outside return 0, stores outside are dropped.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, uint32_t bytes)
 {
@@ -338,7 +333,10 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
     // steps: y of rows t, t-1, t-2; x-differences of rows t, t-1, t-2; y-differences of
     // source rows s = t-1 and s-1.
     float yring[3][C], dxring[3][C], dyring[3][C], pring[3][C], nx[C], nxp[C];
-    Contrib slot[3][C];
+    // Running sums of the output rows in flight (see the gather below): head[c] holds row
+    // s + 1's first two terms, body[c] row s's sum up to its last two terms, and tgy_up[c] /
+    // tgd_up[c] the second-order terms of source row s, which row s + 1 takes next step.
+    float head[C], body[C], tgy_up[C], tgd_up[C];
     DD acc[C];
 
     // Addressing: the lane part of every offset is constant and the row and channel part is
@@ -374,8 +372,10 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
         for (int c = 0; c < C; c++) {
             const int cy = q >> vsh[c];
             const bool cin = cy < crh[c];
+            // The row offset is clamped rather than selected, so no branch splits the step.
+            const int so = min(cy, crh[c] - 1) * crw[c] * 4;
             dst[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                   rpr, (int)(cin ? vpr[c] : a.prob_bytes), cin ? cy * crw[c] * 4 : 0, 0));
+                                                   rpr, (int)(cin ? vpr[c] : a.prob_bytes), so, 0));
         }
     };
 
@@ -383,14 +383,15 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
     for (int c = 0; c < C; c++) {
 #pragma unroll
         for (int j = 0; j < 3; j++) yring[j][c] = dxring[j][c] = dyring[j][c] = pring[j][c] = 0.0f;
+        head[c] = body[c] = tgy_up[c] = tgd_up[c] = 0.0f;
         acc[c] = DD{0.0, 0.0};
     }
     rowload(r0 - 2);
     const int kend = RW + 4;  // steps k = 0 .. RW + 3; step k reads row r0 - 2 + k
 
     // Step k: extrapolate row t = r0 - 2 + k, form its x-differences and the y-differences of
-    // row t - 1, evaluate the contributions of source row t - 1 (k >= 2) and gather output row
-    // t - 3 (k >= 4).  KM = k mod 3 picks the ring slots; PH = min(k, 4) the prologue phase.
+    // row t - 1, evaluate the contributions of source row s = t - 1 (k >= 2) and finish output
+    // row t - 2 (k >= 4).  KM = k mod 3 picks the ring slots; PH = min(k, 4) the prologue phase.
     // From k = 4 on the source row is >= 1, so the top-border tests drop out.
     auto step = [&](const int k, auto KM, auto PH) {
         constexpr int km = decltype(KM)::value, ph = decltype(PH)::value;
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
         for (int c = 0; c < C; c++) yt[c] = fista(nx[c], nxp[c], f);
         if (k + 1 < kend) {
             rowload(t + 1);
-            if constexpr (ph >= 3) probload(t - 1, pring[k2]);  // output row t - 1 of step k + 1
+            if constexpr (ph >= 1) probload(t + 1, pring[k2]);  // the head of step k + 1 (row t + 1)
         }
 #pragma unroll
         for (int c = 0; c < C; c++) dxt[c] = from_right(yt[c]) - yt[c];
@@ -449,32 +450,36 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                     const float n2 = sqrtf(n2acc);
                     rn2 = (valid && n2 != 0.0f) ? recip64(n2) : 0.0;
                 }
-                Contrib *cur = slot[km];
+                float tvs[C], tvr[C], tvd[C], tgs[C], tgx[C], tgy[C], tgd[C];
 #pragma unroll
                 for (int c = 0; c < C; c++) {
                     const float gx = dxprev[c], gy = dys[c];
-                    Contrib kc{0, 0, 0, 0, 0, 0, 0};
-                    kc.tvs = qdiv(atv * -(gx + gy), rn);
-                    kc.tvr = qdiv(atv * gx, rn);
-                    kc.tvd = qdiv(atv * gy, rn);
+                    tvs[c] = qdiv(atv * -(gx + gy), rn);
+                    tvr[c] = qdiv(atv * gx, rn);
+                    tvd[c] = qdiv(atv * gy, rn);
+                    tgs[c] = tgx[c] = tgy[c] = tgd[c] = 0.0f;
                     if (TGV) {
                         const float xx = gxx[c], yy = gyy[c], sm = sym[c];
                         const float sx = sm + xx;
                         // -(2 yy + (2 xx + 2 sm)) == -2 (yy + (sm + xx)) bit for bit: doubling
                         // is exact and commutes with rounding (a subnormal sum is exact).
-                        kc.tgs = atg * qdiv(-2.0f * (yy + sx), rn2);
-                        kc.tgx = atg * qdiv(sx, rn2);
-                        kc.tgy = atg * qdiv(yy + sm, rn2);
-                        kc.tgd = atg * qdiv(-sm, rn2);
+                        tgs[c] = atg * qdiv(-2.0f * (yy + sx), rn2);
+                        tgx[c] = atg * qdiv(sx, rn2);
+                        tgy[c] = atg * qdiv(yy + sm, rn2);
+                        tgd[c] = atg * qdiv(-sm, rn2);
                     }
-                    cur[c] = kc;
                 }
+                // Gather in the reference's order for destination q:
+                //   prob(q); TV from q-W, q-1, q; second-order from q-W, q-W+1, q-1, q, q+1,
+                //   q+W-1, q+W.
+                // A row's sum is built as its terms become known, so only four values per channel
+                // stay live between steps: it starts one step early (prob, then tvd of the row
+                // above: the head), takes its own row's terms with the row above's tgy and tgd
+                // (the body), and ends with the row below's tgd (from the left) and tgy.
                 if constexpr (ph >= 4) {
-                    // Gather into q = s - 1 in the reference's accumulation order.
+                    // Finish output row q = s - 1 and store it with its y.
                     const int q = s - 1;
-                    const Contrib *m1 = slot[k1];  // row q = s - 1
-                    const Contrib *m2 = slot[k2];  // row q - 1 = s - 2
-                    const float *yq = yring[k2];   // y of row q = t - 2
+                    const float *yq = yring[k2];  // y of row q = t - 2
                     const bool qin = q < H;
                     const bool acc_on = out_lane && qin;
                     // Branch-free: a row outside the image moves the lane offset out of range
@@ -483,25 +488,10 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                     const uint32_t qoff = qin ? (uint32_t)q * (uint32_t)W * 4u : 0u;
 #pragma unroll
                     for (int c = 0; c < C; c++) {
-                        const float pv = pring[km][c];
-                        const float left_tvr = from_left(m1[c].tvr);
-                        float gv = 0.0f;
-                        gv += pv;
-                        gv += m2[c].tvd;
-                        gv += left_tvr;
-                        gv += m1[c].tvs;
+                        float gv = body[c];
                         if (TGV) {
-                            const float right_tgd = from_right(m2[c].tgd);
-                            const float left_tgx = from_left(m1[c].tgx);
-                            const float right_tgx = from_right(m1[c].tgx);
-                            const float left_tgd = from_left(cur[c].tgd);
-                            gv += m2[c].tgy;
-                            gv += right_tgd;
-                            gv += left_tgx;
-                            gv += m1[c].tgs;
-                            gv += right_tgx;
-                            gv += left_tgd;
-                            gv += cur[c].tgy;
+                            gv += from_left(tgd[c]);
+                            gv += tgy[c];
                         }
                         const int so = (int)(qoff + (uint32_t)c * pbytes);
                         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), rg, (int)vst, so, 0);
@@ -509,6 +499,33 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                         const float ga = acc_on ? gv : 0.0f;
                         dd_add(acc[c], (double)ga * (double)ga);
                     }
+                }
+                if constexpr (ph >= 3) {
+                    // Body of row s: its own terms and the row above's second-order terms.
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        float gv = head[c];
+                        gv += from_left(tvr[c]);
+                        gv += tvs[c];
+                        if (TGV) {
+                            gv += tgy_up[c];
+                            gv += from_right(tgd_up[c]);
+                            gv += from_left(tgx[c]);
+                            gv += tgs[c];
+                            gv += from_right(tgx[c]);
+                        }
+                        body[c] = gv;
+                    }
+                }
+                // Head of row s + 1 (its prob term, prefetched last step, then row s's tvd).
+#pragma unroll
+                for (int c = 0; c < C; c++) {
+                    float gv = 0.0f;
+                    gv += pring[km][c];
+                    gv += tvd[c];
+                    head[c] = gv;
+                    tgy_up[c] = tgy[c];
+                    tgd_up[c] = tgd[c];
                 }
             }
         }
@@ -518,14 +535,16 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
     step(1, IC<1>{}, IC<1>{});
     step(2, IC<2>{}, IC<2>{});
     step(3, IC<0>{}, IC<3>{});
-    for (int k = 4;; k += 3) {
+    // Whole groups of three steps (no exit inside a group, so values that one step hands to the
+    // next stay in one basic block), then the remaining 0-2 steps.
+    int k = 4;
+    for (; k + 3 <= kend; k += 3) {
         step(k, IC<1>{}, IC<4>{});
-        if (k + 1 >= kend) break;
         step(k + 1, IC<2>{}, IC<4>{});
-        if (k + 2 >= kend) break;
         step(k + 2, IC<0>{}, IC<4>{});
-        if (k + 3 >= kend) break;
     }
+    if (k < kend) step(k, IC<1>{}, IC<4>{});
+    if (k + 1 < kend) step(k + 1, IC<2>{}, IC<4>{});
 
     // Per-workgroup partial sums, in a fixed order: wave c merges channel c across the
     // workgroup's waves lane by lane, then across its lanes; k_finalize reduces the
