@@ -277,6 +277,8 @@ struct IterArgs {
     double *rnorm;           // [3] 1/norm, correctly rounded f64
     double *norm_sq;         // [3]
     uint32_t kg_rows;        // k_gradient output rows per wave
+    uint32_t fuse_norm;      // 1: the projection reduces the norms itself (no k_finalize launch)
+    uint32_t nwg_grad;       // k_gradient workgroups (partials)
 };
 
 // Raw buffer resource over `bytes` bytes: loads outside return 0, stores outside are dropped.
@@ -567,50 +569,105 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
     }
 }
 
-// Gradient norm per channel from the workgroup partials (compute_step.rs:54-58): one
-// workgroup, fixed reduction order, so the result is the same on every run.
+// Gradient norm per channel from the workgroup partials (compute_step.rs:54-58) in a fixed
+// reduction order, so the result is the same on every run and in every workgroup that forms
+// it.  Called by all NF threads of a workgroup; on return (after a barrier) out.norm[c] and
+// out.rnorm[c] hold the results in LDS, and if `writer` is set the workgroup has also stored
+// them, with the squared norm, to the plan's buffers.
 constexpr int NF = 256;
-template <int C>
-__global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigned int nwg, float *norms,
-                                                 double *rnorm, double *norm_sq)
+struct NormShared {
+    DD red[NF / 64][3];
+    float norm[3];
+    double rnorm[3];
+};
+// BATCH partials per thread are loaded before they are merged (k_finalize: 8, to have many
+// loads in flight; a projection kernel: 1, to keep its register budget).  The merge order is
+// the same for every BATCH: thread t takes partials t, t + NF, t + 2 NF, ... and merges a zero
+// pair past the last one up to a multiple of 8 NF.
+template <int C, int BATCH>
+__device__ __forceinline__ void reduce_norms(const double *partials, unsigned int nwg, bool writer, float *norms,
+                                             double *rnorm, double *norm_sq, NormShared &out)
 {
-    __shared__ DD red[NF / 64][3];
+    static_assert(8 % BATCH == 0, "BATCH divides 8");
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     DD s[C];
 #pragma unroll
     for (int c = 0; c < C; c++) s[c] = DD{0.0, 0.0};
+#pragma unroll 1
     for (unsigned int w0 = 0; w0 < nwg; w0 += 8 * NF) {
-        double v[8][2 * C];
+#pragma unroll 1
+        for (int u0 = 0; u0 < 8; u0 += BATCH) {
+            double v[BATCH][2 * C];
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const unsigned int w = w0 + u * NF + tid;
+            for (int u = 0; u < BATCH; u++) {
+                const unsigned int w = w0 + (u0 + u) * NF + tid;
 #pragma unroll
-            for (int k = 0; k < 2 * C; k++) v[u][k] = w < nwg ? partials[(size_t)w * 6 + k] : 0.0;
+                for (int k = 0; k < 2 * C; k++) v[u][k] = w < nwg ? partials[(size_t)w * 6 + k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; u++)
+#pragma unroll
+                for (int c = 0; c < C; c++) s[c] = dd_merge(s[c], DD{v[u][2 * c], v[u][2 * c + 1]});
         }
-#pragma unroll
-        for (int u = 0; u < 8; u++)
-#pragma unroll
-            for (int c = 0; c < C; c++) s[c] = dd_merge(s[c], DD{v[u][2 * c], v[u][2 * c + 1]});
     }
 #pragma unroll
     for (int c = 0; c < C; c++) {
         DD v = s[c];
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) v = dd_merge(v, dd_shfl_xor(v, m));
-        if (lane == 0) red[wave][c] = v;
+        if (lane == 0) out.red[wave][c] = v;
     }
     __syncthreads();
     if (tid < C) {
         const int c = tid;
-        DD v = red[0][c];
+        DD v = out.red[0][c];
 #pragma unroll
-        for (int w = 1; w < NF / 64; w++) v = dd_merge(v, red[w][c]);
+        for (int w = 1; w < NF / 64; w++) v = dd_merge(v, out.red[w][c]);
         const double sum = v.hi + v.lo;
         const float norm = (float)sqrt(sum);
-        norm_sq[c] = sum;
-        norms[c] = norm;
-        rnorm[c] = norm != 0.0f ? 1.0 / (double)norm : 0.0;
+        const double rn = norm != 0.0f ? 1.0 / (double)norm : 0.0;
+        out.norm[c] = norm;
+        out.rnorm[c] = rn;
+        if (writer) {
+            norm_sq[c] = sum;
+            norms[c] = norm;
+            rnorm[c] = rn;
+        }
     }
+    __syncthreads();
+}
+
+// The reduction as a kernel of its own (one workgroup), between k_gradient and the projection.
+template <int C>
+__global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigned int nwg, float *norms,
+                                                 double *rnorm, double *norm_sq)
+{
+    __shared__ NormShared ns;
+    reduce_norms<C, 8>(partials, nwg, true, norms, rnorm, norm_sq, ns);
+}
+
+// A projection kernel's view of the norms: reduced by its own workgroup from the k_gradient
+// partials when the plan runs without k_finalize (a.fuse_norm), else read from the plan.
+// Every thread of the workgroup calls it (the fused reduction has barriers).  FUSE is a
+// template parameter so that the kernels of the separate-reduction plans carry none of it.
+template <int C, bool FUSE>
+__device__ __forceinline__ void projection_norms(const IterArgs &a, NormShared &ns)
+{
+    static_assert(NT == NF, "projection workgroups are NF threads");
+    if constexpr (FUSE)
+        reduce_norms<C, 1>(a.partials, a.nwg_grad, (blockIdx.x | blockIdx.y) == 0, a.norms, a.rnorm, a.norm_sq, ns);
+}
+template <bool FUSE>
+__device__ __forceinline__ float proj_norm(const IterArgs &a, const NormShared &ns, int c)
+{
+    if constexpr (FUSE) return ns.norm[c];
+    else return a.norms[c];
+}
+template <bool FUSE>
+__device__ __forceinline__ double proj_rnorm(const IterArgs &a, const NormShared &ns, int c)
+{
+    if constexpr (FUSE) return ns.rnorm[c];
+    else return a.rnorm[c];
 }
 
 struct ProjShared {
@@ -751,10 +808,11 @@ __device__ __forceinline__ void project_channel(const IterArgs &a, const ChanDev
     __syncthreads();
 }
 
-template <int C>
+template <int C, bool FUSE>
 __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
 {
     __shared__ ProjShared sh;
+    __shared__ NormShared ns;
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     const int W = a.W, H = a.H;
@@ -771,11 +829,12 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
         yv[c] = *reinterpret_cast<const float4 *>(a.y[c] + p);
         gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
     }
+    projection_norms<C, FUSE>(a, ns);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const ChanDev &ch = a.ch[c];
-        const float norm = a.norms[c];
-        const double rn = a.rnorm[c];
+        const float norm = proj_norm<FUSE>(a, ns, c);
+        const double rn = proj_rnorm<FUSE>(a, ns, c);
         // Descent step on the extrapolated image (compute_step.rs:61-65).
         const float ys[4] = {yv[c].x, yv[c].y, yv[c].z, yv[c].w};
         const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
@@ -908,12 +967,13 @@ __device__ __forceinline__ int fused_block(int b, int x0, int y0, int &bx, int &
 
 // k_project for the common layouts (channel modes known at compile time): every
 // channel's blocks go through one DCT pass and one double IDCT pass together.
-template <int C, int M0, int M1, int M2>
+template <int C, int M0, int M1, int M2, bool FUSE>
 __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
 {
     using L = FusedLayout<C, M0, M1, M2>;
     constexpr int TOTB = L::TOTB;
     __shared__ FusedShared<C, M0, M1, M2> sh;
+    __shared__ NormShared ns;
     const int tid = threadIdx.x;
     const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
     const int W = a.W, H = a.H;
@@ -929,6 +989,7 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
         yv[c] = *reinterpret_cast<const float4 *>(a.y[c] + p);
         gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
     }
+    projection_norms<C, FUSE>(a, ns);
     // The coefficients of this thread's row-pass items, loaded now so that their latency
     // overlaps the tile loads (the row pass runs after three barriers).
     constexpr int ITEMS = (TOTB * 8 + NT - 1) / NT;
@@ -952,8 +1013,8 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
     float yd[C][4];
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        const float norm = a.norms[c];
-        const double rn = a.rnorm[c];
+        const float norm = proj_norm<FUSE>(a, ns, c);
+        const double rn = proj_rnorm<FUSE>(a, ns, c);
         const float ys[4] = {yv[c].x, yv[c].y, yv[c].z, yv[c].w};
         const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
 #pragma unroll
@@ -1466,9 +1527,10 @@ __constant__ float kDct8[64] = {
     0x1.8f8b84p-4f, -0x1.1c73b4p-2f, 0x1.a9b662p-2f, -0x1.f6297cp-2f, 0x1.f6297cp-2f, -0x1.a9b662p-2f, 0x1.1c73b4p-2f, -0x1.8f8b84p-4f,
 };
 
-template <int C>
+template <int C, bool FUSE>
 __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
 {
+    __shared__ NormShared ns;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = a.W, H = a.H;
     // The wave's four 8x8 blocks sit side by side in one 8-row band: super-tile blocks (0,0),
@@ -1512,11 +1574,12 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
         gl[c] = *reinterpret_cast<const float4 *>(a.g[c] + pix);
         cl[c] = *reinterpret_cast<const int2 *>(a.ch[c].coef + coef_index(c));
     }
+    projection_norms<C, FUSE>(a, ns);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const ChanDev &ch = a.ch[c];
-        const float norm = a.norms[c];
-        const double rn = a.rnorm[c];
+        const float norm = proj_norm<FUSE>(a, ns, c);
+        const double rn = proj_rnorm<FUSE>(a, ns, c);
         const float ys[4] = {yl[c].x, yl[c].y, yl[c].z, yl[c].w};
         const float gs[4] = {gl[c].x, gl[c].y, gl[c].z, gl[c].w};
         f32x4 xv;
@@ -1687,11 +1750,13 @@ __device__ __forceinline__ void lane_store_prob(const ChanDev &ch, int by, int b
 // within their own lane group, and store nothing.
 
 // Full-resolution channel: 8 lanes per block, block row j = lane >> 3.
-__device__ __forceinline__ void lane_unit_full(const IterArgs &a, int c, int by, int bx, int j, bool valid)
+template <bool FUSE>
+__device__ __forceinline__ void lane_unit_full(const IterArgs &a, const NormShared &ns, int c, int by, int bx, int j,
+                                               bool valid)
 {
     const ChanDev &ch = a.ch[c];
-    const float norm = a.norms[c];
-    const double rn = a.rnorm[c];
+    const float norm = proj_norm<FUSE>(a, ns, c);
+    const double rn = proj_rnorm<FUSE>(a, ns, c);
     const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
     const size_t p = valid ? (size_t)(by * 8 + j) * a.W + bx * 8 : 0;
     const float4 *yp = reinterpret_cast<const float4 *>(a.y[c] + p);
@@ -1719,11 +1784,13 @@ __device__ __forceinline__ void lane_unit_full(const IterArgs &a, int c, int by,
 // 4:2:0 chroma: 16 lanes per block, full-resolution row r = lane >> 2 of the 16-row band; block
 // row j = r / 2 is held twice (rows 2j and 2j + 1, lanes 4 apart), and the upper row's lane
 // stores the block's outputs.
-__device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, int c, int by, int bx, int r, bool valid)
+template <bool FUSE>
+__device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, const NormShared &ns, int c, int by, int bx, int r,
+                                                bool valid)
 {
     const ChanDev &ch = a.ch[c];
-    const float norm = a.norms[c];
-    const double rn = a.rnorm[c];
+    const float norm = proj_norm<FUSE>(a, ns, c);
+    const double rn = proj_rnorm<FUSE>(a, ns, c);
     const int j = r >> 1, sy = r & 1;
     const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
     const size_t p = valid ? (size_t)(by * 16 + r) * a.W + bx * 16 : 0;
@@ -1786,11 +1853,13 @@ constexpr int lane_blocks_per_wave() { return MODE == PM_SUB22 ? 4 : 8; }
 
 // One wave per unit = 8 (full resolution) or 4 (4:2:0 chroma) consecutive blocks of a block row;
 // units of channel 0 first.  u0, u1, u2: cumulative unit counts after channels 0, 1, 2.
-template <int C, int M0, int M1, int M2>
+template <int C, int M0, int M1, int M2, bool FUSE>
 __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t u0, uint32_t u1, uint32_t u2)
 {
+    __shared__ NormShared ns;
     const int lane = threadIdx.x & 63;
     const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
+    projection_norms<C, FUSE>(a, ns);  // before any wave leaves: the fused reduction has barriers
     if (unit >= u2) return;
     auto run = [&](auto MODE_C, auto C_C, uint32_t local) {
         constexpr int MODE = decltype(MODE_C)::value, c = decltype(C_C)::value;
@@ -1798,8 +1867,8 @@ __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t 
         const ChanDev &ch = a.ch[c];
         const uint32_t per_row = (ch.bw + BPW - 1) / BPW;
         const int by = (int)(local / per_row), bx = (int)(local % per_row) * BPW + lane % BPW;
-        if constexpr (MODE == PM_SUB22) lane_unit_sub22(a, c, by, bx, lane / BPW, bx < (int)ch.bw);
-        else lane_unit_full(a, c, by, bx, lane / BPW, bx < (int)ch.bw);
+        if constexpr (MODE == PM_SUB22) lane_unit_sub22<FUSE>(a, ns, c, by, bx, lane / BPW, bx < (int)ch.bw);
+        else lane_unit_full<FUSE>(a, ns, c, by, bx, lane / BPW, bx < (int)ch.bw);
     };
     if (unit < u0) run(IC<M0>{}, IC<0>{}, unit);
     else if (C > 1 && unit < u1) run(IC<M1>{}, IC<1>{}, unit - u0);
@@ -1820,6 +1889,7 @@ struct artefact_hip_plan {
     int layout = 0;  // LAYOUT_*: which k_project variant runs
     int layout_fused = 0;  // the fused layout this plan's channels allow (LAYOUT_GENERIC if none)
     int lane_project = 0;  // fused layouts: 1 = k_project_lane (no LDS), 0 = k_project_fused
+    int fuse_norm = 0;     // 1: the projection workgroups reduce the norms (no k_finalize launch)
     uint32_t lane_units[3] = {0, 0, 0};  // k_project_lane: cumulative units (one wave's blocks) per channel
     uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
     float qmean[3] = {0, 0, 0};                   // mean quantisation step per channel (warm start)
@@ -1922,6 +1992,8 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
     a.rnorm = pl->rnorm;
     a.norm_sq = pl->norm_sq;
     a.kg_rows = (uint32_t)pl->rows;
+    a.fuse_norm = pl->fuse_norm ? 1u : 0u;
+    a.nwg_grad = pl->grid_g.x * pl->grid_g.y;
     return a;
 }
 
@@ -1929,12 +2001,13 @@ template <int C, bool TGV>
 void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
     hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, s, a);
-    const unsigned int nwg = pl->grid_g.x * pl->grid_g.y;
-    hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials, nwg,
-                       pl->norms, pl->rnorm, pl->norm_sq);
+    if (!a.fuse_norm)
+        hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials, a.nwg_grad,
+                           pl->norms, pl->rnorm, pl->norm_sq);
 }
 
-// Gradient sweep + norm: k_gradient then k_finalize.
+// Gradient sweep + norm: k_gradient then k_finalize (unless the projection that follows
+// reduces the norms itself, a.fuse_norm).
 void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
     switch (pl->C * 2 + (pl->tgv ? 1 : 0)) {
@@ -1947,12 +2020,13 @@ void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t
     }
 }
 
-void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
+template <bool F>
+void launch_project_f(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
     if (pl->precision == ARTEFACT_PRECISION_BF16X3) {
-        if (pl->C == 1) hipLaunchKernelGGL(k_project_mfma<1>, pl->grid_p, dim3(NT), 0, s, a);
-        else if (pl->C == 2) hipLaunchKernelGGL(k_project_mfma<2>, pl->grid_p, dim3(NT), 0, s, a);
-        else hipLaunchKernelGGL(k_project_mfma<3>, pl->grid_p, dim3(NT), 0, s, a);
+        if (pl->C == 1) hipLaunchKernelGGL((k_project_mfma<1, F>), pl->grid_p, dim3(NT), 0, s, a);
+        else if (pl->C == 2) hipLaunchKernelGGL((k_project_mfma<2, F>), pl->grid_p, dim3(NT), 0, s, a);
+        else hipLaunchKernelGGL((k_project_mfma<3, F>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     }
     if (pl->lane_project && pl->layout != LAYOUT_GENERIC) {
@@ -1960,34 +2034,42 @@ void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t 
         const uint32_t u0 = pl->lane_units[0], u1 = pl->lane_units[1], u2 = pl->lane_units[2];
         switch (pl->layout) {
         case LAYOUT_Y:
-            hipLaunchKernelGGL((k_project_lane<1, PM_FULL, PM_FULL, PM_FULL>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            hipLaunchKernelGGL((k_project_lane<1, PM_FULL, PM_FULL, PM_FULL, F>), grid, dim3(NT), 0, s, a, u0, u1, u2);
             return;
         case LAYOUT_444:
-            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_FULL, PM_FULL>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_FULL, PM_FULL, F>), grid, dim3(NT), 0, s, a, u0, u1, u2);
             return;
         default:
-            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_SUB22, PM_SUB22>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_SUB22, PM_SUB22, F>), grid, dim3(NT), 0, s, a, u0, u1,
+                               u2);
             return;
         }
     }
     switch (pl->layout) {
     case LAYOUT_Y:
-        hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL, F>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     case LAYOUT_444:
-        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_FULL, PM_FULL, F>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     case LAYOUT_420:
-        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_SUB22, PM_SUB22>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_SUB22, PM_SUB22, F>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     default: break;
     }
     if (pl->C == 1)
-        hipLaunchKernelGGL(k_project<1>, pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project<1, F>), pl->grid_p, dim3(NT), 0, s, a);
     else if (pl->C == 2)
-        hipLaunchKernelGGL(k_project<2>, pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project<2, F>), pl->grid_p, dim3(NT), 0, s, a);
     else
-        hipLaunchKernelGGL(k_project<3>, pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project<3, F>), pl->grid_p, dim3(NT), 0, s, a);
+}
+
+// The projection kernel for the plan's layout, precision and norm mode (a.fuse_norm).
+void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
+{
+    if (a.fuse_norm) launch_project_f<true>(pl, a, s);
+    else launch_project_f<false>(pl, a, s);
 }
 
 int launch_iterations(artefact_hip_plan *pl, hipStream_t s)
@@ -2086,7 +2168,7 @@ int run_colour(hipStream_t stream, const float *y, const float *cb, const float 
 // Candidates are timed on the plan itself (eager launches, HIP events around a few
 // iterations after a fresh init) and the winner is cached for the process.
 struct TuneChoice {
-    int rows, layout, lane;
+    int rows, layout, lane, fuse;
     float ms;
 };
 std::mutex g_tune_mu;
@@ -2144,7 +2226,7 @@ void drop_graph(artefact_hip_plan *pl)
 int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
 {
     const std::string key = tune_key(pl);
-    TuneChoice best{pl->rows, pl->layout, pl->lane_project, -1.0f};
+    TuneChoice best{pl->rows, pl->layout, pl->lane_project, pl->fuse_norm, -1.0f};
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -2183,13 +2265,28 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
                 if (t < 0.0f) return ARTEFACT_E_HIP;
                 t = std::min(t, time_loop(pl, nt));
                 if (t < 0.0f) return ARTEFACT_E_HIP;
-                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, pj.first, pj.second, t};
+                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, pj.first, pj.second, pl->fuse_norm, t};
             }
+        // The winner once more with the norm reduction moved into its projection workgroups
+        // (no k_finalize launch): fewer launches per iteration, but every projection workgroup
+        // reads all k_gradient partials.
+        if (!getenv("ARTEFACT_FUSE_NORM")) {
+            pl->layout = best.layout;
+            pl->lane_project = best.lane;
+            pl->fuse_norm = 1 - best.fuse;
+            set_rows(pl, best.rows);
+            float t = time_loop(pl, nt);
+            if (t < 0.0f) return ARTEFACT_E_HIP;
+            t = std::min(t, time_loop(pl, nt));
+            if (t < 0.0f) return ARTEFACT_E_HIP;
+            if (t < best.ms) best = TuneChoice{best.rows, best.layout, best.lane, pl->fuse_norm, t};
+        }
         std::lock_guard<std::mutex> lock(g_tune_mu);
         g_tune_cache[key] = best;
     }
     pl->layout = best.layout;
     pl->lane_project = best.lane;
+    pl->fuse_norm = best.fuse;
     set_rows(pl, best.rows);
     drop_graph(pl);
     pl->initialized = false;
@@ -2353,6 +2450,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     // layouts (4096x3072 4:4:4: 144 vs 194 us), the LDS kernel for 4:2:0 (2048x1536: 36 vs 41 us).
     pl->lane_project = 1;  // measured faster than k_project_fused on every fused layout
     if (const char *e = getenv("ARTEFACT_PROJECT_KERNEL")) pl->lane_project = strcmp(e, "lds") != 0;
+    if (const char *e = getenv("ARTEFACT_FUSE_NORM")) pl->fuse_norm = atoi(e) != 0;
     if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
         if (atoi(e)) pl->layout = LAYOUT_GENERIC;
     ok = ok && hipMemcpyAsync(pl->quant, quant.data(), quant.size() * sizeof(float), hipMemcpyHostToDevice,
@@ -2484,7 +2582,9 @@ int artefact_hip_plan_first_gradient(artefact_hip_plan *pl, float *const *out)
     if (!pl || !out) return ARTEFACT_E_ARGUMENT;
     if (!pl->initialized) return ARTEFACT_E_STATE;
     DeviceGuard guard(pl->device);
-    launch_gradient(pl, make_args(pl, pl->cur, 0.0f, false), pl->stream);
+    IterArgs ga = make_args(pl, pl->cur, 0.0f, false);
+    ga.fuse_norm = 0;  // the norms are part of this call's result
+    launch_gradient(pl, ga, pl->stream);
     bool ok = hipGetLastError() == hipSuccess;
     for (uint32_t c = 0; c < pl->C; c++) {
         if (!out[c]) return ARTEFACT_E_ARGUMENT;
@@ -2747,7 +2847,9 @@ int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
     // coefficients, the state the loop carries into its next gradient.
     bool ok = hipMemsetAsync(pl->norms, 0, 3 * sizeof(float), pl->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(pl->rnorm, 0, 3 * sizeof(double), pl->stream) == hipSuccess;
-    launch_project(pl, make_args(pl, 0, 0.0f, false), pl->stream);
+    IterArgs pa = make_args(pl, 0, 0.0f, false);
+    pa.fuse_norm = 0;  // the zeroed norms, not a reduction of stale partials
+    launch_project(pl, pa, pl->stream);
     ok = ok && hipMemcpyAsync(pl->buf[0], pl->buf[1], (size_t)pl->C * pl->N * sizeof(float),
                               hipMemcpyDeviceToDevice, pl->stream) == hipSuccess;
     ok = ok && hipGetLastError() == hipSuccess;
@@ -2760,13 +2862,14 @@ int artefact_hip_plan_tune(artefact_hip_plan *pl, artefact_tuning *out)
 {
     if (!pl) return ARTEFACT_E_ARGUMENT;
     DeviceGuard guard(pl->device);
-    TuneChoice t{pl->rows, pl->layout, pl->lane_project, -1.0f};
+    TuneChoice t{pl->rows, pl->layout, pl->lane_project, pl->fuse_norm, -1.0f};
     const int rc = tune_plan(pl, &t);
     if (rc) return rc;
     if (out) {
         out->kg_rows = (uint32_t)t.rows;
         out->project_kernel = t.layout == LAYOUT_GENERIC ? 0u : (t.lane ? 2u : 1u);
         out->ms_per_iteration = t.ms;
+        out->fused_norm = (uint32_t)t.fuse;
     }
     return ARTEFACT_OK;
 }

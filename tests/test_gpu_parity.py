@@ -284,3 +284,46 @@ def test_bit_exact_at_configured_sizes(name, iters):
     ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, norm_mode="f64")
     for c, (a, b) in enumerate(zip(gpu, ref)):
         assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
+
+
+@pytest.mark.parametrize("name,iters,env", [
+    ("tgv_420_208x144", 7, {}),
+    ("tgv_420_96x64", 5, {"ARTEFACT_PROJECT_KERNEL": "lds"}),
+    ("tgv_444_72x40", 6, {}),
+    ("tv_luma_64x64", 6, {"ARTEFACT_GENERIC_PROJECT": "1"}),
+    ("tgv_422_80x40", 5, {}),
+    ("tgv_420_2048", 2, {}),
+])
+def test_fused_norm_bit_exact(name, iters, env, monkeypatch):
+    """Norms reduced by every projection workgroup (no k_finalize launch) give the same bits
+    as the separate reduction: against the oracle with the f64 norm, for each projection kernel."""
+    monkeypatch.setenv("ARTEFACT_FUSE_NORM", "1")
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    wl = synth.workload(name, seed=31)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case fused-norm {name} iters={iters} {env}", flush=True)
+    gpu = run_gpu(wl, iters)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, norm_mode="f64")
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
+
+
+def test_fused_norm_precision_form_unchanged(monkeypatch):
+    """The BF16 MFMA projection with the norms reduced in its own workgroups: the same bits as
+    with the separate reduction, and the plan's last norms agree."""
+    wl = synth.workload("tgv_444_72x40", seed=37)
+    W, H = wl["max_w"], wl["max_h"]
+    outs, norms = [], []
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("ARTEFACT_FUSE_NORM", fuse)
+        print(f"case fused-norm precision fuse={fuse}", flush=True)
+        with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], 6, W, H, mode=wl["mode"]) as plan:
+            plan.set_precision("bf16x3")
+            plan.init()
+            plan.iterate()
+            outs.append(plan.download())
+            norms.append(plan.last_norms())
+    for a, b in zip(outs[0], outs[1]):
+        assert bits_equal(a, b)
+    assert bits_equal(norms[0], norms[1])
