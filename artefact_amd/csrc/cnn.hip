@@ -68,7 +68,9 @@ constexpr ParamLayout PL = param_layout();
 
 // MODE_RELU: + bias, ReLU, bf16 out.  MODE_LINEAR: no bias, bf16 out (backward data).
 // MODE_FINAL: + bias, channel 0 only, f32 out = residual feature + conv (the prediction).
-enum { MODE_RELU = 0, MODE_LINEAR = 1, MODE_FINAL = 2 };
+// MODE_FINAL_IMAGE: MODE_FINAL with the residual formed from the decoded plane (x0 / 64, what
+// k_pack_input stores) and the prediction written in image units (x 64), for inference.
+enum { MODE_RELU = 0, MODE_LINEAR = 1, MODE_FINAL = 2, MODE_FINAL_IMAGE = 3 };
 
 // Output tile of one workgroup: 64 pixels (wave w: pixels 16w .. 16w+15) x RT rows.  The input
 // tile (RT + 2 rows x 66 pixels x CI channels, zero outside the plane) is staged in LDS once;
@@ -81,16 +83,53 @@ constexpr int TILE_PX = 66;
 template <int CI>
 constexpr int lds_px_bytes() { return CI * 2 + 8; }  // padded pixel stride (bank spread), a multiple of 8
 
-template <int CI, int CO, int MODE>
+// FEAT_IN: the input is the decoded plane itself (f32, `in` cast back) and the kernel forms the
+// 16 feature channels while staging its tile: bf16(x0 / 64), bf16(qstep / 64), then zeros, as
+// k_pack_input does.
+template <int CI, int CO, int MODE, bool FEAT_IN = false>
 __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, const bf16x4 *__restrict__ wpk,
                                              const float *__restrict__ bias, uint16_t *__restrict__ out,
                                              float *__restrict__ pred, const float *__restrict__ resid, int H,
-                                             int W)
+                                             int W, float qstep)
 {
+    static_assert(!FEAT_IN || CI == 16, "features fill one 16-channel chunk");
     constexpr int KC = CI / 16, MH = CO / 16, PXB = lds_px_bytes<CI>();
     __shared__ __attribute__((aligned(16))) uint8_t tile[(RT + 2) * TILE_PX * PXB];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int b = blockIdx.z, y0 = blockIdx.y * RT, x0 = blockIdx.x * 64;
+    // Stage the input tile: 16-byte units (8 channels), consecutive threads on consecutive units.
+    // Every load of the thread is issued before the first store, so the tile costs one memory
+    // round trip; a unit past the tile reloads the last one and is not stored.
+    constexpr int UPX = CI / 8, UNITS = (RT + 2) * TILE_PX * UPX, PER = (UNITS + CT - 1) / CT;
+    uint4 tv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int u = min((int)threadIdx.x + i * CT, UNITS - 1);
+        const int part = u % UPX, px = (u / UPX) % TILE_PX, r = u / (UPX * TILE_PX);
+        const int gy = y0 - 1 + r, gx = x0 - 1 + px;
+        const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
+        const size_t pix = ((size_t)b * H + (ok ? gy : 0)) * W + (ok ? gx : 0);
+        uint4 v;
+        if constexpr (FEAT_IN) {
+            const float x0 = reinterpret_cast<const float *>(in)[pix];
+            const uint32_t f01 = (uint32_t)f2bf(x0 * FEAT) | ((uint32_t)f2bf(qstep * FEAT) << 16);
+            v = make_uint4(part == 0 ? f01 : 0u, 0u, 0u, 0u);
+        } else {
+            v = *reinterpret_cast<const uint4 *>(in + pix * CI + part * 8);
+        }
+        tv[i] = ok ? v : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        const int u = (int)threadIdx.x + i * CT;
+        if (u >= UNITS) break;
+        const int part = u % UPX, px = (u / UPX) % TILE_PX, r = u / (UPX * TILE_PX);
+        // The padded pixel stride is a multiple of 8 bytes, not 16: two 8-byte stores.
+        uint2 *dst = reinterpret_cast<uint2 *>(tile + (r * TILE_PX + px) * PXB + part * 16);
+        dst[0] = make_uint2(tv[i].x, tv[i].y);
+        dst[1] = make_uint2(tv[i].z, tv[i].w);
+    }
+    // The weights (all 9 taps in registers) are loaded after the tile's loads are consumed.
     bf16x4 wr[9][KC][MH];
 #pragma unroll
     for (int t = 0; t < 9; t++)
@@ -98,19 +137,6 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
         for (int kc = 0; kc < KC; kc++)
 #pragma unroll
             for (int mh = 0; mh < MH; mh++) wr[t][kc][mh] = wpk[((t * KC + kc) * MH + mh) * 64 + lane];
-    // Stage the input tile: 16-byte units (8 channels), consecutive threads on consecutive units.
-    constexpr int UPX = CI / 8, UNITS = (RT + 2) * TILE_PX * UPX;
-    for (int u = threadIdx.x; u < UNITS; u += CT) {
-        const int part = u % UPX, px = (u / UPX) % TILE_PX, r = u / (UPX * TILE_PX);
-        const int gy = y0 - 1 + r, gx = x0 - 1 + px;
-        const bool ok = gy >= 0 && gy < H && gx >= 0 && gx < W;
-        const size_t g = (((size_t)b * H + (ok ? gy : 0)) * W + (ok ? gx : 0)) * CI + part * 8;
-        const uint4 v = *reinterpret_cast<const uint4 *>(in + g);
-        // The padded pixel stride is a multiple of 8 bytes, not 16: two 8-byte stores.
-        uint2 *dst = reinterpret_cast<uint2 *>(tile + (r * TILE_PX + px) * PXB + part * 16);
-        dst[0] = ok ? make_uint2(v.x, v.y) : make_uint2(0, 0);
-        dst[1] = ok ? make_uint2(v.z, v.w) : make_uint2(0, 0);
-    }
     __syncthreads();
     const int n = lane & 15, kq = (lane >> 4) * 4;
     f32x4 acc[RT][MH];
@@ -150,6 +176,10 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
         const size_t p = ((size_t)b * H + y) * W + px;
         if (MODE == MODE_FINAL) {
             if (kq == 0) pred[p] = resid[p] + (acc[ro][0][0] + bias[0]);
+            continue;
+        }
+        if (MODE == MODE_FINAL_IMAGE) {
+            if (kq == 0) pred[p] = (resid[p] * FEAT + (acc[ro][0][0] + bias[0])) * 64.0f;
             continue;
         }
 #pragma unroll
@@ -311,11 +341,6 @@ __global__ __launch_bounds__(CT) void k_pack_input(const float *x0, const float 
 }
 
 // Prediction back to image units.
-__global__ __launch_bounds__(CT) void k_unpack_pred(const float *pred, size_t n, float *out)
-{
-    const size_t i = (size_t)blockIdx.x * CT + threadIdx.x;
-    if (i < n) out[i] = pred[i] * 64.0f;
-}
 
 // MSE loss gradient in feature units: g = 2 (pred - target / 64) / n into channel 0 of a
 // 16-channel bf16 gradient; per-workgroup partial sums of the squared error (f64).
@@ -422,12 +447,12 @@ void repack(artefact_cnn *net)
     }
 }
 
-template <int CI, int CO, int MODE>
+template <int CI, int CO, int MODE, bool FEAT_IN = false>
 void conv(const uint16_t *in, const bf16x4 *w, const float *bias, uint16_t *out, float *pred,
-          const float *resid, int B, int H, int W, hipStream_t s)
+          const float *resid, int B, int H, int W, hipStream_t s, float qstep = 0.0f)
 {
-    hipLaunchKernelGGL((k_conv<CI, CO, MODE>), dim3((W + 63) / 64, (H + RT - 1) / RT, B), dim3(CT), 0, s, in, w,
-                       bias, out, pred, resid, H, W);
+    hipLaunchKernelGGL((k_conv<CI, CO, MODE, FEAT_IN>), dim3((W + 63) / 64, (H + RT - 1) / RT, B), dim3(CT), 0, s,
+                       in, w, bias, out, pred, resid, H, W, qstep);
 }
 
 // Forward pass on the features in act[0] / resid; the prediction (feature units) in pred.
@@ -508,15 +533,22 @@ int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const floa
     if (!net || !planes || !qsteps || !out || !nplanes || nplanes > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
     const size_t plane = (size_t)h * w;
     if (!ensure_workspace(net, plane)) return ARTEFACT_E_HIP;
-    // One upload of every plane's step (net->qstep holds 64 floats).
-    bool ok = hipMemcpyAsync(net->qstep, qsteps, (size_t)nplanes * 4, hipMemcpyHostToDevice, s) == hipSuccess;
+    // The first layer forms the features from the plane while staging its tiles and the last
+    // writes the prediction in image units, so no packing kernels run (same arithmetic).
+    bool ok = true;
     for (uint32_t i = 0; i < nplanes && ok; i++) {
-        hipLaunchKernelGGL(k_pack_input, dim3(blocks(plane)), dim3(CT), 0, s, planes[i],
-                           (const float *)(net->qstep + i), plane, 1, net->act[0], net->resid);
-        forward(net, 1, (int)h, (int)w, s);
-        hipLaunchKernelGGL(k_unpack_pred, dim3(blocks(plane)), dim3(CT), 0, s, (const float *)net->pred, plane,
-                           out[i]);
-        ok = ok && hipGetLastError() == hipSuccess;
+        conv<16, 32, MODE_RELU, true>(reinterpret_cast<const uint16_t *>(planes[i]), net->wf[0],
+                                      net->params + PL.b[0], net->act[1], nullptr, nullptr, 1, (int)h, (int)w, s,
+                                      qsteps[i]);
+        conv<32, 32, MODE_RELU>(net->act[1], net->wf[1], net->params + PL.b[1], net->act[2], nullptr, nullptr, 1,
+                                (int)h, (int)w, s);
+        conv<32, 32, MODE_RELU>(net->act[2], net->wf[2], net->params + PL.b[2], net->act[3], nullptr, nullptr, 1,
+                                (int)h, (int)w, s);
+        conv<32, 32, MODE_RELU>(net->act[3], net->wf[3], net->params + PL.b[3], net->act[4], nullptr, nullptr, 1,
+                                (int)h, (int)w, s);
+        conv<32, 16, MODE_FINAL_IMAGE>(net->act[4], net->wf[4], net->params + PL.b[4], nullptr, out[i], planes[i], 1,
+                                       (int)h, (int)w, s);
+        ok = hipGetLastError() == hipSuccess;
     }
     return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
@@ -568,13 +600,18 @@ int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *qsteps
                          uint32_t w, float *pred)
 {
     if (!net || !x0 || !qsteps || !pred || !batch || batch > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
-    if (!upload_batch(net, x0, qsteps, nullptr, (int)batch, (int)h, (int)w, false)) return ARTEFACT_E_HIP;
-    const size_t px = (size_t)batch * h * w;
-    forward(net, (int)batch, (int)h, (int)w, net->stream);
-    hipLaunchKernelGGL(k_unpack_pred, dim3(blocks(px)), dim3(CT), 0, net->stream, (const float *)net->pred, px,
-                       net->x0);
-    bool ok = hipGetLastError() == hipSuccess;
-    ok = ok && hipMemcpyAsync(pred, net->x0, px * 4, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
+    const size_t plane = (size_t)h * w, px = batch * plane;
+    if (!ensure_workspace(net, px)) return ARTEFACT_E_HIP;
+    // The inference path warm_init runs, plane by plane.
+    const float *planes[64];
+    float *outs[64];
+    for (uint32_t i = 0; i < batch; i++) {
+        planes[i] = net->x0 + i * plane;
+        outs[i] = net->pred + i * plane;
+    }
+    bool ok = hipMemcpyAsync(net->x0, x0, px * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
+    ok = ok && cnn_predict_planes(net, planes, qsteps, batch, h, w, outs, net->stream) == ARTEFACT_OK;
+    ok = ok && hipMemcpyAsync(pred, net->pred, px * 4, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
     return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
