@@ -134,30 +134,49 @@ def qstep(coef) -> float:
     return float(np.asarray(coef.quant_table, dtype=np.float64).mean())
 
 
+def _encode_crop(job):
+    """One synthetic crop, encoded (runs in a worker process; no GPU use)."""
+    size, img_seed, quality = job
+    return synth.encode(synth.synth_rgb(size, size, img_seed), subsampling="420", quality=quality)
+
+
 def training_pairs(count: int, size: int = 256, seed: int = 0, target_iterations: int = 300,
-                   qualities=(20, 90), weight: float = 0.3, pweight: float = 0.001, log=print):
+                   qualities=(20, 90), weight: float = 0.3, pweight: float = 0.001, log=print,
+                   workers: int = 8):
     """``count`` synthetic 4:2:0 crops of size x size at qualities drawn uniformly in the
     range; every channel plane (grid resolution) is one sample.  Returns (x0, q, target) with
-    x0 the decoded start plane and target the plain TGV loop's output, all float32."""
+    x0 the decoded start plane and target the plain TGV loop's output, all float32.  The
+    crops are synthesised and encoded by ``workers`` spawned processes (host work only); the
+    start images and targets are solved on the GPU by this process."""
+    import multiprocessing
+
     rng = np.random.default_rng(seed)
-    imgs, starts, qs = [], [], []
-    for i in range(count):
-        quality = int(rng.integers(qualities[0], qualities[1] + 1))
-        coefs = synth.encode(synth.synth_rgb(size, size, seed * 100003 + i), subsampling="420", quality=quality)
-        imgs.append(BatchImage(coefs, weight, [pweight] * 3, target_iterations, size, size, "tgv"))
-        with Plan(coefs, weight, [pweight] * 3, 0, size, size) as plan:
-            plan.init()
-            starts += plan.download()
-        qs += [qstep(c) for c in coefs]
-        if log and (i + 1) % 512 == 0:
-            log(f"encoded {i + 1} of {count} crops", flush=True)
-    targets = []
-    for c0 in range(0, count, 64):  # graphs of at most 64 images
-        with Batch(imgs[c0:c0 + 64], streams=4, tune=False) as b:
-            b.run()
-            targets += [p for i in range(len(imgs[c0:c0 + 64])) for p in b.download(i)]
-    shape = (len(starts), size, size)
-    return (np.stack(starts).reshape(shape), np.asarray(qs, np.float32), np.stack(targets).reshape(shape))
+    jobs = [(size, seed * 100003 + i, int(rng.integers(qualities[0], qualities[1] + 1))) for i in range(count)]
+    x0 = np.empty((3 * count, size, size), np.float32)
+    target = np.empty_like(x0)
+    qs = np.empty(3 * count, np.float32)
+    chunk = 64  # graphs of at most 64 images
+    ctx = multiprocessing.get_context("spawn")
+    with ctx.Pool(max(1, workers)) as pool:
+        encoded = pool.imap(_encode_crop, jobs, chunksize=16)
+        for c0 in range(0, count, chunk):
+            coefs_list = [next(encoded) for _ in range(min(chunk, count - c0))]
+            imgs = []
+            for k, coefs in enumerate(coefs_list):
+                i = c0 + k
+                imgs.append(BatchImage(coefs, weight, [pweight] * 3, target_iterations, size, size, "tgv"))
+                with Plan(coefs, weight, [pweight] * 3, 0, size, size) as plan:
+                    plan.init()
+                    x0[3 * i:3 * i + 3] = np.stack(plan.download()).reshape(3, size, size)
+                qs[3 * i:3 * i + 3] = [qstep(c) for c in coefs]
+            with Batch(imgs, streams=4, tune=False) as b:
+                b.run()
+                for k in range(len(imgs)):
+                    i = c0 + k
+                    target[3 * i:3 * i + 3] = np.stack(b.download(k)).reshape(3, size, size)
+            if log and ((c0 + chunk) % 2048 == 0 or c0 + chunk >= count):
+                log(f"solved {min(c0 + chunk, count)} of {count} crops", flush=True)
+    return x0, qs, target
 
 
 def train(net: WarmStartNet, x0, q, target, steps: int, batch: int = 16, lr: float = 1e-3, seed: int = 0,
@@ -218,17 +237,18 @@ def main(argv=None) -> int:
     import argparse
 
     ap = argparse.ArgumentParser(description="train the warm-start CNN on solver-generated pairs")
-    ap.add_argument("--crops", type=int, default=1024)
+    ap.add_argument("--crops", type=int, default=20000)
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=30000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--target-iterations", type=int, default=300)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--workers", type=int, default=8, help="processes that synthesise and encode the crops")
     ap.add_argument("--out", default=WEIGHTS_FILE)
     args = ap.parse_args(argv)
     t0 = time.perf_counter()
-    x0, q, target = training_pairs(args.crops, args.size, args.seed, args.target_iterations)
+    x0, q, target = training_pairs(args.crops, args.size, args.seed, args.target_iterations, workers=args.workers)
     t_data = time.perf_counter() - t0
     print(f"pairs: {x0.shape[0]} planes of {args.size}x{args.size} in {t_data:.1f} s", flush=True)
     net = WarmStartNet(init_params(args.seed))

@@ -1,5 +1,5 @@
 """Learned warm start with the committed network (artefact_amd/warmstart_weights.npz, trained
-on the GPU by `python -m artefact_amd.warmstart` on 4096 solver-generated 256x256 crops).
+on the GPU by `python -m artefact_amd.warmstart` on 20 000 solver-generated 256x256 crops).
 
 What the network gains: on crops outside its training set, K plain iterations from the
 CNN's projected prediction end at a lower objective and closer to the converged (300
