@@ -581,9 +581,9 @@ struct NormShared {
     double rnorm[3];
 };
 // BATCH partials per thread are loaded before they are merged (k_finalize: 8, to have many
-// loads in flight; a projection kernel: 1, to keep its register budget).  The merge order is
-// the same for every BATCH: thread t takes partials t, t + NF, t + 2 NF, ... and merges a zero
-// pair past the last one up to a multiple of 8 NF.
+// loads in flight; a projection kernel: 4, to keep its register budget).  The result is the
+// same for every BATCH: thread t merges partials t, t + NF, t + 2 NF, ... in that order
+// (a zero pair past the last one changes nothing).
 template <int C, int BATCH>
 __device__ __forceinline__ void reduce_norms(const double *partials, unsigned int nwg, bool writer, float *norms,
                                              double *rnorm, double *norm_sq, NormShared &out)
@@ -597,6 +597,9 @@ __device__ __forceinline__ void reduce_norms(const double *partials, unsigned in
     for (unsigned int w0 = 0; w0 < nwg; w0 += 8 * NF) {
 #pragma unroll 1
         for (int u0 = 0; u0 < 8; u0 += BATCH) {
+            // A batch past the last partial would merge only zero pairs, and merging a zero pair
+            // into a normalised double-double returns it unchanged, so it is skipped.
+            if (w0 + (unsigned int)u0 * NF >= nwg) break;
             double v[BATCH][2 * C];
 #pragma unroll
             for (int u = 0; u < BATCH; u++) {
@@ -650,12 +653,15 @@ __global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigne
 // partials when the plan runs without k_finalize (a.fuse_norm), else read from the plan.
 // Every thread of the workgroup calls it (the fused reduction has barriers).  FUSE is a
 // template parameter so that the kernels of the separate-reduction plans carry none of it.
-template <int C, bool FUSE>
+// BATCH: partials loaded per thread at a time (1 keeps the MFMA kernel's register budget; the
+// lane kernel takes 2 at no register cost).
+template <int C, bool FUSE, int BATCH = 1>
 __device__ __forceinline__ void projection_norms(const IterArgs &a, NormShared &ns)
 {
     static_assert(NT == NF, "projection workgroups are NF threads");
     if constexpr (FUSE)
-        reduce_norms<C, 1>(a.partials, a.nwg_grad, (blockIdx.x | blockIdx.y) == 0, a.norms, a.rnorm, a.norm_sq, ns);
+        reduce_norms<C, BATCH>(a.partials, a.nwg_grad, (blockIdx.x | blockIdx.y) == 0, a.norms, a.rnorm, a.norm_sq,
+                               ns);
 }
 template <bool FUSE>
 __device__ __forceinline__ float proj_norm(const IterArgs &a, const NormShared &ns, int c)
@@ -1859,7 +1865,7 @@ __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t 
     __shared__ NormShared ns;
     const int lane = threadIdx.x & 63;
     const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
-    projection_norms<C, FUSE>(a, ns);  // before any wave leaves: the fused reduction has barriers
+    projection_norms<C, FUSE, 2>(a, ns);  // before any wave leaves: the fused reduction has barriers
     if (unit >= u2) return;
     auto run = [&](auto MODE_C, auto C_C, uint32_t local) {
         constexpr int MODE = decltype(MODE_C)::value, c = decltype(C_C)::value;
