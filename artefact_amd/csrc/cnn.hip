@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/artefact_hip.h"
@@ -392,11 +393,306 @@ unsigned blocks(size_t n) { return (unsigned)((n + CT - 1) / CT); }
 
 }  // namespace
 
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Inference in one kernel: the five layers stream down a column band.
+//
+// A workgroup owns SB = 64 output columns and RH output rows of one plane.  Wave 0 stages the
+// input features and runs layer 1; waves 1-6 run layers 2-4, one wave per half of a layer's
+// output channels; wave 7 runs layer 5 and writes the prediction.  Each step, every layer
+// produces one row, over 80 columns starting 5 - l columns left of the band (the next layers
+// read 64 + 2 (5 - l) of them).  Rows pass from layer to layer through LDS rings of two rows:
+// what a layer writes in step t the next one reads in step t + 1, with one barrier per step.
+// Layers 2-5 keep three output rows of accumulators: taking in input row j adds the dy = 0 taps
+// of output row j + 1, the dy = 1 taps of row j and the dy = 2 taps of row j - 1, which is then
+// complete.  Every output therefore sums its taps in the order (dy, dx), each tap one
+// v_mfma_f32_16x16x32_bf16 over all 32 input channels.  Layer 1 packs its 9 taps x 2 features
+// into one K = 32 operand.  Outputs outside the plane are stored as zeros: the next layer's
+// zero padding.  Rows computed above or below the ones the band needs (pipeline fill and
+// drain) only feed rows that are not needed either.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+constexpr int SB = 64;                            // output columns per band
+constexpr int ST = 5;                             // 16-pixel tiles per layer row (80 columns)
+constexpr int SPX = 82;                           // ring row: 80 columns + the 2 a 3-tap read reaches past
+constexpr int SSTR = 80;                          // bytes per ring pixel: 32 bf16 channels + 16 padding
+constexpr int SROW = SPX * SSTR;                  // one ring row
+constexpr int SRING = 2 * SROW;                   // one activation ring (rows alternate by step)
+constexpr int SFEAT = 4 * SPX * 4;                // feature ring: 4 rows of (f0, q) bf16 pairs
+constexpr int SLDS = 4 * SRING + SFEAT;           // rings of layers 1-4, then the features
+constexpr int SFILL = 12;                         // steps from a band's first input row to its first output row
+// Packed weights, one u32x4 (8 bf16) per lane: layer 1 [mh][lane]; layers 2-4 [tap][mh][lane];
+// layer 5 [tap][lane].
+constexpr int SW_L2 = 128, SW_MID = 9 * 2 * 64, SW_L5 = SW_L2 + 3 * SW_MID, SW_TOTAL = SW_L5 + 9 * 64;
+static_assert(SLDS % 16 == 0, "rings are cleared in 16-byte units");
+
+__device__ __forceinline__ f32x4 mfma32(u32x4 a, u32x4 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                    0, 0);
+}
+
+// bias + ReLU + bf16 of one accumulator (channels 4g .. 4g+3 of a 16-channel half), packed.
+__device__ __forceinline__ uint2 relu_pack(f32x4 acc, const float b[4], bool keep)
+{
+    uint16_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float r = acc[k] + b[k];
+        r = r > 0.0f ? r : 0.0f;
+        v[k] = f2bf(r);
+    }
+    const uint2 p = make_uint2((uint32_t)v[0] | ((uint32_t)v[1] << 16), (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+    return keep ? p : make_uint2(0u, 0u);
+}
+
+// Layers 2-4 (l = 2, 3, 4): one wave per half (mh) of the 32 output channels, 5 tiles.
+__device__ __forceinline__ void stream_mid(uint8_t *lds, const u32x4 *__restrict__ wts, const float *__restrict__ params,
+                                           int l, int mh, int bx0, int y0, int H, int W, int T, int lane)
+{
+    const int n = lane & 15, g = lane >> 4;
+    u32x4 w[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) w[t] = wts[SW_L2 + (l - 2) * SW_MID + (t * 2 + mh) * 64 + lane];
+    float bias[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) bias[k] = params[PL.b[l - 1] + mh * 16 + 4 * g + k];
+    const uint8_t *in = lds + (l - 2) * SRING;
+    uint8_t *outr = lds + (l - 1) * SRING + (mh * 16 + 4 * g) * 2;
+    const int xo = bx0 - (5 - l);   // plane column of ring pixel 0 of this layer's output
+    const int j0 = y0 - 1 - 2 * l;  // input row taken in at step 0
+    f32x4 acc[3][ST];
+#pragma unroll
+    for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int i = 0; i < ST; i++) acc[s][i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    auto step = [&](int t, auto PC) {
+        constexpr int P = decltype(PC)::value;
+        constexpr int S0 = (P + 1) % 3, S1 = P, S2 = (P + 2) % 3;  // rows j + 1, j, j - 1
+        const uint8_t *rin = in + ((t + 1) & 1) * SROW + g * 16;
+        uint8_t *rout = outr + (t & 1) * SROW;
+        const int y = j0 + t - 1;
+        const bool rowok = y >= 0 && y < H;
+#pragma unroll
+        for (int i = 0; i < ST; i++) {
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                const u32x4 bv = *reinterpret_cast<const u32x4 *>(rin + (i * 16 + n + dx) * SSTR);
+                acc[S0][i] = mfma32(w[dx], bv, acc[S0][i]);
+                acc[S1][i] = mfma32(w[3 + dx], bv, acc[S1][i]);
+                acc[S2][i] = mfma32(w[6 + dx], bv, acc[S2][i]);
+            }
+            // Row j - 1 of this tile is complete: out while the next tiles' products run.
+            const int x = xo + i * 16 + n;
+            *reinterpret_cast<uint2 *>(rout + (i * 16 + n) * SSTR) =
+                relu_pack(acc[S2][i], bias, rowok && x >= 0 && x < W);
+            acc[S2][i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        __syncthreads();
+    };
+    for (int t = 0; t < T; t += 3) {
+        step(t, IC<0>{});
+        step(t + 1, IC<1>{});
+        step(t + 2, IC<2>{});
+    }
+}
+
+// Layer 1 and the feature staging: one wave.
+__device__ __forceinline__ void stream_first(uint8_t *lds, const float *__restrict__ x0, const u32x4 *__restrict__ wts,
+                                             const float *__restrict__ params, int bx0, int y0, int H, int W, int T,
+                                             float qstep, int lane)
+{
+    const int n = lane & 15, g = lane >> 4;
+    u32x4 w1[2];
+#pragma unroll
+    for (int mh = 0; mh < 2; mh++) w1[mh] = wts[mh * 64 + lane];
+    float b1[2][4];
+#pragma unroll
+    for (int mh = 0; mh < 2; mh++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) b1[mh][k] = params[PL.b[0] + mh * 16 + 4 * g + k];
+    uint32_t *feat = reinterpret_cast<uint32_t *>(lds + 4 * SRING);  // [4][SPX]
+    uint8_t *r1 = lds;                                               // layer 1's output ring
+    const uint32_t qbits = (uint32_t)f2bf(qstep * FEAT) << 16;
+
+    // Feature row f is plane row y0 - 5 + f; ring pixel p is plane column bx0 - 5 + p.
+    auto load_row = [&](int f, float v[2]) {
+        const int y = min(max(y0 - 5 + f, 0), H - 1);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int x = min(max(bx0 - 5 + lane + 64 * k, 0), W - 1);
+            v[k] = x0[(size_t)y * W + x];
+        }
+    };
+    auto store_row = [&](int f, const float v[2]) {
+        const int y = y0 - 5 + f;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int p = lane + 64 * k;
+            const int x = bx0 - 5 + p;
+            const bool ok = y >= 0 && y < H && x >= 0 && x < W;
+            if (p < SPX) feat[(f & 3) * SPX + p] = ok ? ((uint32_t)f2bf(v[k] * FEAT) | qbits) : 0u;
+        }
+    };
+    float pf[2];
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        load_row(f, pf);
+        store_row(f, pf);
+    }
+    load_row(3, pf);
+
+    // B operand: lane (n, g) holds k = 8g .. 8g+7, i.e. taps 4g .. 4g+3 with their two features
+    // (k = 2 tap + feature); taps past 8 are zero.
+    int tdy[4], tdx[4];
+    bool tok[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int tap = 4 * g + q;
+        tok[q] = tap < 9;
+        tdy[q] = tap / 3;
+        tdx[q] = tap % 3;
+    }
+    for (int t = 0; t < T; t++) {
+        // Output row y0 - 4 + t from feature rows t .. t + 2.
+        const int y = y0 - 4 + t;
+        const bool rowok = y >= 0 && y < H;
+        uint8_t *rout = r1 + (t & 1) * SROW;
+#pragma unroll
+        for (int i = 0; i < ST; i++) {
+            u32x4 bv;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t v = feat[((t + tdy[q]) & 3) * SPX + i * 16 + n + tdx[q]];
+                bv[q] = tok[q] ? v : 0u;
+            }
+            const int x = bx0 - 4 + i * 16 + n;
+            const bool ok = rowok && x >= 0 && x < W;
+#pragma unroll
+            for (int mh = 0; mh < 2; mh++) {
+                const f32x4 a = mfma32(w1[mh], bv, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+                *reinterpret_cast<uint2 *>(rout + (i * 16 + n) * SSTR + (mh * 16 + 4 * g) * 2) =
+                    relu_pack(a, b1[mh], ok);
+            }
+        }
+        // Next feature row in (loaded one step ahead), the one after that requested.
+        store_row(t + 3, pf);
+        load_row(t + 4, pf);
+        __syncthreads();
+    }
+}
+
+// Layer 5 and the output: one wave.  Takes in layer 4's row y0 - 11 + t at step t; output row
+// y0 - 12 + t is then complete.
+__device__ __forceinline__ void stream_last(uint8_t *lds, const float *__restrict__ x0, const u32x4 *__restrict__ wts,
+                                            const float *__restrict__ params, float *__restrict__ out, int bx0, int y0,
+                                            int RH, int H, int W, int T, int lane)
+{
+    const int n = lane & 15, g = lane >> 4;
+    u32x4 w5[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) w5[t] = wts[SW_L5 + t * 64 + lane];
+    const float b5 = params[PL.b[4]];
+    const uint8_t *r4 = lds + 3 * SRING + g * 16;  // layer 4's output ring
+    f32x4 acc5[3][ST - 1];
+#pragma unroll
+    for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int i = 0; i < ST - 1; i++) acc5[s][i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    auto step = [&](int t, auto PC) {
+        constexpr int P = decltype(PC)::value;
+        constexpr int S0 = (P + 1) % 3, S1 = P, S2 = (P + 2) % 3;
+        const uint8_t *rin = r4 + ((t + 1) & 1) * SROW;
+        const int y = y0 - 12 + t;
+        const bool rowok = y >= y0 && y < y0 + RH && y < H;
+#pragma unroll
+        for (int i = 0; i < ST - 1; i++) {
+#pragma unroll
+            for (int dx = 0; dx < 3; dx++) {
+                const u32x4 bv = *reinterpret_cast<const u32x4 *>(rin + (i * 16 + n + dx) * SSTR);
+                acc5[S0][i] = mfma32(w5[dx], bv, acc5[S0][i]);
+                acc5[S1][i] = mfma32(w5[3 + dx], bv, acc5[S1][i]);
+                acc5[S2][i] = mfma32(w5[6 + dx], bv, acc5[S2][i]);
+            }
+            const int x = bx0 + i * 16 + n;
+            if (g == 0 && rowok && x < W) {
+                const size_t p = (size_t)y * W + x;
+                out[p] = (x0[p] * FEAT + (acc5[S2][i][0] + b5)) * 64.0f;
+            }
+            acc5[S2][i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        __syncthreads();
+    };
+    for (int t = 0; t < T; t += 3) {
+        step(t, IC<0>{});
+        step(t + 1, IC<1>{});
+        step(t + 2, IC<2>{});
+    }
+}
+
+// Grid: (bands of SB columns, segments of RH rows); RH a multiple of 3.  Eight waves: wave 0
+// layer 1, waves 1-6 layers 2-4 (two channel halves each), wave 7 layer 5.
+constexpr int SCT = 512;
+__global__ __launch_bounds__(SCT, 4) void k_cnn_stream(const float *__restrict__ x0, const u32x4 *__restrict__ wts,
+                                                       const float *__restrict__ params, float *__restrict__ out,
+                                                       int H, int W, int RH, float qstep)
+{
+    __shared__ __attribute__((aligned(16))) uint8_t lds[SLDS];
+    for (int i = threadIdx.x; i < SLDS / 16; i += SCT) reinterpret_cast<u32x4 *>(lds)[i] = u32x4{0u, 0u, 0u, 0u};
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int bx0 = blockIdx.x * SB, y0 = blockIdx.y * RH, T = RH + SFILL;
+    if (wave == 0) stream_first(lds, x0, wts, params, bx0, y0, H, W, T, qstep, lane);
+    else if (wave == 7) stream_last(lds, x0, wts, params, out, bx0, y0, RH, H, W, T, lane);
+    else stream_mid(lds, wts, params, 2 + (wave - 1) / 2, (wave - 1) & 1, bx0, y0, H, W, T, lane);
+}
+
+// Weights of all five layers into the streaming kernel's operand layouts.
+__global__ __launch_bounds__(CT) void k_pack_stream(const float *__restrict__ params, u32x4 *__restrict__ dst)
+{
+    const int i = blockIdx.x * CT + threadIdx.x;
+    if (i >= SW_TOTAL) return;
+    const int lane = i & 63, m16 = lane & 15, g = lane >> 4;
+    float v[8];
+    if (i < SW_L2) {  // layer 1, [mh][lane]: k = 2 tap + feature
+        const int m = (i >> 6) * 16 + m16;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = 8 * g + e, tap = k >> 1, ch = k & 1;
+            v[e] = tap < 9 ? params[PL.w[0] + (m * 2 + ch) * 9 + tap] : 0.0f;
+        }
+    } else if (i < SW_L5) {  // layers 2-4, [tap][mh][lane]
+        const int r = i - SW_L2, l = 1 + r / SW_MID, tap = (r % SW_MID) / 128, m = ((r % 128) >> 6) * 16 + m16;
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = params[PL.w[l] + (m * 32 + 8 * g + e) * 9 + tap];
+    } else {  // layer 5, [tap][lane]: output channel 0
+        const int tap = (i - SW_L5) >> 6;
+#pragma unroll
+        for (int e = 0; e < 8; e++) v[e] = m16 == 0 ? params[PL.w[4] + (8 * g + e) * 9 + tap] : 0.0f;
+    }
+    u32x4 o;
+#pragma unroll
+    for (int j = 0; j < 4; j++) o[j] = (uint32_t)f2bf(v[2 * j]) | ((uint32_t)f2bf(v[2 * j + 1]) << 16);
+    dst[i] = o;
+}
+
+}  // namespace
+
 struct artefact_cnn {
     int device = 0;
     hipStream_t stream = nullptr;
     float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;  // PL.total each
     bf16x4 *wf[NLAYER] = {}, *wb[NLAYER] = {};  // forward / backward-data operands
+    u32x4 *ws = nullptr;                        // operands of the streaming inference kernel
+    bool ws_stale = true;                       // params changed since ws was packed
+    int cus = 0;                                // compute units of the device
     int adam_t = 0;
     // workspace for a batch of B planes of H x W
     size_t cap_px = 0;
@@ -438,6 +734,7 @@ bool ensure_workspace(artefact_cnn *net, size_t px)
 
 void repack(artefact_cnn *net)
 {
+    net->ws_stale = true;
     for (int l = 0; l < NLAYER; l++) {
         const int nf = 9 * (CINP[l] / 16) * (COUTP[l] / 16) * 64;
         hipLaunchKernelGGL(k_pack_weights, dim3(blocks(nf)), dim3(CT), 0, net->stream, net->params + PL.w[l],
@@ -525,17 +822,45 @@ bool upload_batch(artefact_cnn *net, const float *x0, const float *q, const floa
     return ok && hipGetLastError() == hipSuccess;
 }
 
+// The layered inference path (one kernel per layer, activations through HBM), kept for
+// comparison: selected by ARTEFACT_CNN_LAYERED=1.
+bool layered_requested()
+{
+    const char *e = getenv("ARTEFACT_CNN_LAYERED");
+    return e && e[0] == '1';
+}
+
 }  // namespace
 
 int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const float *qsteps, uint32_t nplanes,
                        uint32_t h, uint32_t w, float *const *out, hipStream_t s)
 {
     if (!net || !planes || !qsteps || !out || !nplanes || nplanes > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
+    bool ok = true;
+    if (!layered_requested()) {
+        if (net->ws_stale) {
+            hipLaunchKernelGGL(k_pack_stream, dim3(blocks(SW_TOTAL)), dim3(CT), 0, s, (const float *)net->params,
+                               net->ws);
+            net->ws_stale = false;
+        }
+        // Bands of SB columns; row segments sized for about two workgroups per compute unit (a
+        // multiple of 3 rows, at least 12, so that the pipeline fill stays a modest share).
+        const int nbx = ((int)w + SB - 1) / SB;
+        int nby = std::max(1, (2 * std::max(net->cus, 1) + nbx - 1) / nbx);
+        int rh = ((int)h + nby - 1) / nby;
+        rh = std::max(12, (rh + 2) / 3 * 3);
+        nby = ((int)h + rh - 1) / rh;
+        for (uint32_t i = 0; i < nplanes && ok; i++) {
+            hipLaunchKernelGGL(k_cnn_stream, dim3(nbx, nby), dim3(SCT), 0, s, planes[i], (const u32x4 *)net->ws,
+                               (const float *)net->params, out[i], (int)h, (int)w, rh, qsteps[i]);
+            ok = hipGetLastError() == hipSuccess;
+        }
+        return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
+    }
     const size_t plane = (size_t)h * w;
     if (!ensure_workspace(net, plane)) return ARTEFACT_E_HIP;
     // The first layer forms the features from the plane while staging its tiles and the last
     // writes the prediction in image units, so no packing kernels run (same arithmetic).
-    bool ok = true;
     for (uint32_t i = 0; i < nplanes && ok; i++) {
         conv<16, 32, MODE_RELU, true>(reinterpret_cast<const uint16_t *>(planes[i]), net->wf[0],
                                       net->params + PL.b[0], net->act[1], nullptr, nullptr, 1, (int)h, (int)w, s,
@@ -573,6 +898,8 @@ int artefact_cnn_create(const float *params, artefact_cnn **out)
         const size_t nf = (size_t)9 * (CINP[l] / 16) * (COUTP[l] / 16) * 64 * sizeof(bf16x4);
         ok = hipMalloc(&net->wf[l], nf) == hipSuccess && hipMalloc(&net->wb[l], nf) == hipSuccess;
     }
+    ok = ok && hipMalloc(&net->ws, (size_t)SW_TOTAL * sizeof(u32x4)) == hipSuccess;
+    ok = ok && hipDeviceGetAttribute(&net->cus, hipDeviceAttributeMultiprocessorCount, net->device) == hipSuccess;
     net->wgroups = 512;
     ok = ok && hipMalloc(&net->part, (size_t)net->wgroups * 10 * 32 * 32 * 4) == hipSuccess;
     ok = ok && hipMemcpyAsync(net->params, params, pb, hipMemcpyHostToDevice, net->stream) == hipSuccess;
@@ -689,7 +1016,7 @@ void artefact_cnn_destroy(artefact_cnn *net)
     if (net->stream) (void)hipStreamSynchronize(net->stream);
     std::vector<void *> bufs{net->params, net->grads, net->m, net->v, net->qstep, net->part, net->g32a, net->g32b,
                              net->g16, net->resid, net->pred, net->x0, net->target, net->lpart, net->ds_x0,
-                             net->ds_target, net->ds_q};
+                             net->ds_target, net->ds_q, net->ws};
     for (int l = 0; l < NLAYER; l++) {
         bufs.push_back(net->wf[l]);
         bufs.push_back(net->wb[l]);

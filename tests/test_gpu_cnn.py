@@ -1,4 +1,5 @@
-"""Warm-start CNN kernels (csrc/cnn.hip) against the CPU reference (oracle/cnn.py).
+"""Warm-start CNN kernels (csrc/cnn.hip) against the CPU reference (oracle/cnn.py), and the
+streaming inference kernel against the layered one.
 
 BF16 activations and weights with f32 accumulation are not bit-identical to the float64
 reference, so the checks are relative: the predicted residual within 3 % (relative L2), each
@@ -84,3 +85,27 @@ def test_adam_lowers_the_loss():
     net.close()
     print(f"loss {losses[0]:.5g} -> {losses[-1]:.5g}")
     assert losses[-1] < 0.25 * losses[0], (losses[0], losses[-1])
+
+
+@pytest.mark.parametrize("H,W", [(24, 48), (70, 100), (200, 136)])
+def test_streaming_kernel_matches_layered(H, W, monkeypatch):
+    """Inference runs all five layers in one streaming kernel (v_mfma_f32_16x16x32_bf16, each
+    tap over all 32 channels at once).  The layered kernels (16x16x16, two channel halves per
+    tap) stay behind ARTEFACT_CNN_LAYERED=1.  Both round to bf16 between layers after
+    different MFMA groupings of the same sums, so they agree to bf16 precision, not bit for
+    bit: the residuals within 1 % (relative L2) and, at every pixel, within 2 % of the
+    residual's range.  The sizes give partial bands and tiles and several row segments."""
+    flat, _, _ = params_with_bias(11)
+    x0, q, _ = batch(12, B=2, H=H, W=W)
+    print(f"case cnn streaming vs layered {H}x{W}", flush=True)
+    net = warmstart.WarmStartNet(flat)
+    got = net.forward(x0, q)
+    monkeypatch.setenv("ARTEFACT_CNN_LAYERED", "1")
+    ref = net.forward(x0, q)
+    net.close()
+    r_got, r_ref = got - x0, ref - x0
+    rel = np.linalg.norm(r_got - r_ref) / np.linalg.norm(r_ref)
+    mx = np.max(np.abs(r_got - r_ref)) / (np.max(r_ref) - np.min(r_ref))
+    print(f"relative L2 {rel:.3g}, max {mx:.3g} of the range")
+    assert rel <= 0.01
+    assert mx <= 0.02
