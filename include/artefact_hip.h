@@ -217,7 +217,9 @@ typedef struct artefact_cnn artefact_cnn;
 uint32_t artefact_cnn_param_count(void);
 int artefact_cnn_create(const float *params, artefact_cnn **out_net);
 int artefact_cnn_get_params(artefact_cnn *net, float *out_params);
-/* Prediction (image units) for `batch` host planes of h x w with one quantisation step each. */
+/* Prediction (image units) for `batch` host planes of h x w with one quantisation step each,
+ * through the inference path of artefact_hip_plan_warm_init: one streaming kernel for all five
+ * layers (ARTEFACT_CNN_LAYERED=1 in the environment selects one kernel per layer instead). */
 int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *qsteps, uint32_t batch, uint32_t h,
                          uint32_t w, float *out_pred);
 /* Mean squared error (in units of 1/64) and its parameter gradients for a host batch; w must
