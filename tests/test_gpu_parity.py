@@ -327,3 +327,17 @@ def test_fused_norm_precision_form_unchanged(monkeypatch):
     for a, b in zip(outs[0], outs[1]):
         assert bits_equal(a, b)
     assert bits_equal(norms[0], norms[1])
+
+
+def test_golden_fixtures_bit_exact():
+    """The committed golden fixtures (tests/golden: 4:2:0, 4:4:4, 4:2:2, 4:4:0 and luma-only
+    TV, partial tiles) through the C ABI: the stored bits of the oracle's planes."""
+    from golden_cases import loop_cases
+
+    for case in loop_cases():
+        print(f"case golden {case['name']} iters={case['iterations']}", flush=True)
+        W, H = case["max_w"], case["max_h"]
+        gpu = artefact_amd.compute(len(case["coefs"]), case["coefs"], case["weight"], case["pweight"],
+                                   case["iterations"], W, H, W * H, mode=case["mode"])
+        for c, (a, b) in enumerate(zip(gpu, case["planes"])):
+            assert bits_equal(a, b), f"{case['name']} channel {c}: max abs diff {np.abs(a - b).max()}"
