@@ -122,18 +122,36 @@ def copy_calibration(nbytes=256 << 20, reps=4):
     return {"bytes_read_per_copy": nbytes, "copy_GBps": round(2 * nbytes * reps / (ms.value * 1e-3) / 1e9, 1)}
 
 
-def cpu_baseline(wl, iters):
+def host_cpu():
+    """(CPU model name, CPUs this process may run on)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(wl, iters, runs=3):
     import oracle
 
     W, H = wl["max_w"], wl["max_h"]
-    _, secs = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, return_time=True)
+    secs = min(oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, return_time=True)[1]
+               for _ in range(runs))
+    model, ncpu = host_cpu()
     return {
         "value": W * H * iters / secs / 1e6,
         "unit": "Mpixel*it/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{describe(wl)} geometry, {iters} iterations, loop only ({secs:.2f} s), single-threaded C restatement "
-                  f"of pipeline_scalar (oracle/oracle.c, -O2 -ffp-contract=off)",
+        "cpu_model": model,
+        "host_cpus": ncpu,
+        "sample": f"{describe(wl)} geometry, {iters} iterations, loop only, best of {runs} runs ({secs:.2f} s), "
+                  f"single-threaded C restatement of pipeline_scalar (oracle/oracle.c, -O2 -ffp-contract=off)",
     }
 
 
