@@ -16,6 +16,7 @@ ARTEFACT_MODE_TGV = 0
 ARTEFACT_MODE_TV = 1
 ARTEFACT_PRECISION_F32 = 0
 ARTEFACT_PRECISION_BF16X3 = 1
+ARTEFACT_PRECISION_F32_FAST = 2
 
 ERRORS = {
     -1: "invalid argument",

@@ -138,8 +138,11 @@ class Plan:
         return out
 
     def set_precision(self, precision: str) -> None:
-        """"f32" (default, the reference's arithmetic) or "bf16x3" (MFMA projection transforms)."""
-        code = {"f32": _lib.ARTEFACT_PRECISION_F32, "bf16x3": _lib.ARTEFACT_PRECISION_BF16X3}[precision]
+        """"f32" (default, the reference's arithmetic bit for bit), "bf16x3" (MFMA projection
+        transforms) or "f32_fast" (hardware square root and reciprocal in the gradient sweep,
+        within a few ulp per term)."""
+        code = {"f32": _lib.ARTEFACT_PRECISION_F32, "bf16x3": _lib.ARTEFACT_PRECISION_BF16X3,
+                "f32_fast": _lib.ARTEFACT_PRECISION_F32_FAST}[precision]
         _lib.check(self._lib.artefact_hip_plan_set_precision(self._h, code), "artefact_hip_plan_set_precision")
 
     def tune(self) -> dict:

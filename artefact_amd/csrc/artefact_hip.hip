@@ -295,8 +295,12 @@ using IC = std::integral_constant<int, N>;
 // four-step prologue, then a loop unrolled by three so that the three-row ring of
 // contributions sits in fixed registers.  The host sizes the strips so that the whole grid
 // is resident at once (one round of waves), which keeps the 2-row halo cost per strip low.
-template <int C, bool TGV>
-__global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
+// FAST (ARTEFACT_PRECISION_F32_FAST): the same gather, with the per-source arithmetic relaxed:
+// v_sqrt_f32 for the joint norms, one v_rcp_f32 per norm and products (alpha / n) * term in
+// place of the correctly rounded (alpha * term) / n, and a plain f64 sum of g^2.  Each term is
+// then within a few ulp of the exact one instead of equal to it.
+template <int C, bool TGV, bool FAST = false>
+__global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
 {
     __shared__ DD lred[WAVES][3][64];
 
@@ -445,30 +449,56 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                 // a NaN reciprocal where IEEE division gives a / inf = 0; it needs squared
                 // differences above 3.4e38, which int16 coefficients and u16 tables cannot
                 // produce.)
-                const float n = sqrtf(nacc);
-                const double rn = (valid && n != 0.0f) ? recip64(n) : 0.0;
-                double rn2 = 0.0;
-                if (TGV) {
-                    const float n2 = sqrtf(n2acc);
-                    rn2 = (valid && n2 != 0.0f) ? recip64(n2) : 0.0;
-                }
                 float tvs[C], tvr[C], tvd[C], tgs[C], tgx[C], tgy[C], tgd[C];
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    const float gx = dxprev[c], gy = dys[c];
-                    tvs[c] = qdiv(atv * -(gx + gy), rn);
-                    tvr[c] = qdiv(atv * gx, rn);
-                    tvd[c] = qdiv(atv * gy, rn);
-                    tgs[c] = tgx[c] = tgy[c] = tgd[c] = 0.0f;
+                if constexpr (FAST) {
+                    const float n = __builtin_amdgcn_sqrtf(nacc);
+                    const float ar = (valid && n != 0.0f) ? atv * __builtin_amdgcn_rcpf(n) : 0.0f;
+                    float gr = 0.0f;
                     if (TGV) {
-                        const float xx = gxx[c], yy = gyy[c], sm = sym[c];
-                        const float sx = sm + xx;
-                        // -(2 yy + (2 xx + 2 sm)) == -2 (yy + (sm + xx)) bit for bit: doubling
-                        // is exact and commutes with rounding (a subnormal sum is exact).
-                        tgs[c] = atg * qdiv(-2.0f * (yy + sx), rn2);
-                        tgx[c] = atg * qdiv(sx, rn2);
-                        tgy[c] = atg * qdiv(yy + sm, rn2);
-                        tgd[c] = atg * qdiv(-sm, rn2);
+                        const float n2 = __builtin_amdgcn_sqrtf(n2acc);
+                        gr = (valid && n2 != 0.0f) ? atg * __builtin_amdgcn_rcpf(n2) : 0.0f;
+                    }
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        const float gx = dxprev[c], gy = dys[c];
+                        tvs[c] = ar * -(gx + gy);
+                        tvr[c] = ar * gx;
+                        tvd[c] = ar * gy;
+                        tgs[c] = tgx[c] = tgy[c] = tgd[c] = 0.0f;
+                        if (TGV) {
+                            const float xx = gxx[c], yy = gyy[c], sm = sym[c];
+                            const float sx = sm + xx;
+                            tgs[c] = gr * (-2.0f * (yy + sx));
+                            tgx[c] = gr * sx;
+                            tgy[c] = gr * (yy + sm);
+                            tgd[c] = gr * -sm;
+                        }
+                    }
+                } else {
+                    const float n = sqrtf(nacc);
+                    const double rn = (valid && n != 0.0f) ? recip64(n) : 0.0;
+                    double rn2 = 0.0;
+                    if (TGV) {
+                        const float n2 = sqrtf(n2acc);
+                        rn2 = (valid && n2 != 0.0f) ? recip64(n2) : 0.0;
+                    }
+#pragma unroll
+                    for (int c = 0; c < C; c++) {
+                        const float gx = dxprev[c], gy = dys[c];
+                        tvs[c] = qdiv(atv * -(gx + gy), rn);
+                        tvr[c] = qdiv(atv * gx, rn);
+                        tvd[c] = qdiv(atv * gy, rn);
+                        tgs[c] = tgx[c] = tgy[c] = tgd[c] = 0.0f;
+                        if (TGV) {
+                            const float xx = gxx[c], yy = gyy[c], sm = sym[c];
+                            const float sx = sm + xx;
+                            // -(2 yy + (2 xx + 2 sm)) == -2 (yy + (sm + xx)) bit for bit: doubling
+                            // is exact and commutes with rounding (a subnormal sum is exact).
+                            tgs[c] = atg * qdiv(-2.0f * (yy + sx), rn2);
+                            tgx[c] = atg * qdiv(sx, rn2);
+                            tgy[c] = atg * qdiv(yy + sm, rn2);
+                            tgd[c] = atg * qdiv(-sm, rn2);
+                        }
                     }
                 }
                 // Gather in the reference's order for destination q:
@@ -499,7 +529,8 @@ __global__ __launch_bounds__(NT, 4) void k_gradient(const IterArgs a)
                         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), rg, (int)vst, so, 0);
                         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yq[c]), ry, (int)vst, so, 0);
                         const float ga = acc_on ? gv : 0.0f;
-                        dd_add(acc[c], (double)ga * (double)ga);
+                        if constexpr (FAST) acc[c].hi = __builtin_fma((double)ga, (double)ga, acc[c].hi);
+                        else dd_add(acc[c], (double)ga * (double)ga);
                     }
                 }
                 if constexpr (ph >= 3) {
@@ -2006,7 +2037,10 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
 template <int C, bool TGV>
 void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, s, a);
+    if (pl->precision == ARTEFACT_PRECISION_F32_FAST)
+        hipLaunchKernelGGL((k_gradient<C, TGV, true>), pl->grid_g, dim3(NT), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, s, a);
     if (!a.fuse_norm)
         hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials, a.nwg_grad,
                            pl->norms, pl->rnorm, pl->norm_sq);
@@ -2105,16 +2139,22 @@ void launch_init(artefact_hip_plan *pl, hipStream_t s)
     }
 }
 
-// The k_gradient instance for (C, TGV), for occupancy queries.
-const void *gradient_kernel(uint32_t C, bool tgv)
+// The k_gradient instance for (C, TGV, FAST), for occupancy queries.
+const void *gradient_kernel(uint32_t C, bool tgv, bool fast)
 {
-    switch (C * 2 + (tgv ? 1 : 0)) {
-    case 2: return (const void *)k_gradient<1, false>;
-    case 3: return (const void *)k_gradient<1, true>;
-    case 4: return (const void *)k_gradient<2, false>;
-    case 5: return (const void *)k_gradient<2, true>;
-    case 6: return (const void *)k_gradient<3, false>;
-    default: return (const void *)k_gradient<3, true>;
+    switch (C * 4 + (tgv ? 2 : 0) + (fast ? 1 : 0)) {
+    case 4: return (const void *)k_gradient<1, false>;
+    case 5: return (const void *)k_gradient<1, false, true>;
+    case 6: return (const void *)k_gradient<1, true>;
+    case 7: return (const void *)k_gradient<1, true, true>;
+    case 8: return (const void *)k_gradient<2, false>;
+    case 9: return (const void *)k_gradient<2, false, true>;
+    case 10: return (const void *)k_gradient<2, true>;
+    case 11: return (const void *)k_gradient<2, true, true>;
+    case 12: return (const void *)k_gradient<3, false>;
+    case 13: return (const void *)k_gradient<3, false, true>;
+    case 14: return (const void *)k_gradient<3, true>;
+    default: return (const void *)k_gradient<3, true, true>;
     }
 }
 
@@ -2124,9 +2164,9 @@ const void *gradient_kernel(uint32_t C, bool tgv)
 // workgroup, stacked) fits the device's resident workgroups at once.  An override
 // (ARTEFACT_KG_ROWS) is honoured.
 // Resident k_gradient workgroups per CU and the CU count (4 and 256 if the queries fail).
-void gradient_residency(int device, uint32_t C, bool tgv, int &per_cu, int &cus)
+void gradient_residency(int device, uint32_t C, bool tgv, bool fast, int &per_cu, int &cus)
 {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gradient_kernel(C, tgv), NT, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gradient_kernel(C, tgv, fast), NT, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1 ||
         cus < 1) {
         per_cu = 4;
@@ -2144,14 +2184,14 @@ int rows_for_workgroups(uint32_t W, uint32_t H, uint64_t wgs)
     return (int)std::max(H, 1u);
 }
 
-int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv)
+int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv, bool fast)
 {
     if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
         const int r = atoi(e);
         if (r >= 1 && r <= 4096) return r;
     }
     int per_cu = 0, cus = 0;
-    gradient_residency(device, C, tgv, per_cu, cus);
+    gradient_residency(device, C, tgv, fast, per_cu, cus);
     return rows_for_workgroups(W, H, (uint64_t)per_cu * cus);
 }
 
@@ -2183,7 +2223,7 @@ std::map<std::string, TuneChoice> g_tune_cache;
 std::string tune_key(const artefact_hip_plan *pl)
 {
     std::string k = "d" + std::to_string(pl->device) + " " + std::to_string(pl->W) + "x" + std::to_string(pl->H) +
-                    " C" + std::to_string(pl->C) + (pl->tgv ? " tgv" : " tv");
+                    " C" + std::to_string(pl->C) + (pl->tgv ? " tgv" : " tv") + " p" + std::to_string(pl->precision);
     for (uint32_t c = 0; c < pl->C; c++)
         k += " " + std::to_string(pl->ch[c].rw) + "x" + std::to_string(pl->ch[c].rh) + "/" +
              std::to_string(pl->ch[c].hs) + std::to_string(pl->ch[c].vs);
@@ -2262,7 +2302,8 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
             // per_cu - 1 or per_cu - 2 workgroups: the sweep is VALU-bound per SIMD, so its time
             // follows the most loaded CU's (workgroups x strip steps).
             int per_cu = 0, cus = 0;
-            gradient_residency(pl->device, pl->C, pl->tgv, per_cu, cus);
+            gradient_residency(pl->device, pl->C, pl->tgv, pl->precision == ARTEFACT_PRECISION_F32_FAST, per_cu,
+                               cus);
             std::vector<int> cand{r0 + 1, r0 + 2, r0 - 1, r0 - 2, 2 * r0};
             for (int k = per_cu - 1; k >= std::max(1, per_cu - 2); k--) {
                 const int rk = rows_for_workgroups(pl->W, pl->H, (uint64_t)k * cus);
@@ -2386,7 +2427,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         pl->M[c] = (size_t)chs[c].rounded_w * chs[c].rounded_h;
         mtot += pl->M[c];
     }
-    pl->rows = pick_rows(pl->device, pl->W, pl->H, pl->C, pl->tgv);
+    pl->rows = pick_rows(pl->device, pl->W, pl->H, pl->C, pl->tgv, false);
     pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * pl->rows - 1) / (WAVES * pl->rows));
     pl->grid_p = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
     // Partials for the largest grid any strip height (>= 2) can give, so that the tuner can
@@ -2822,7 +2863,9 @@ int artefact_hip_process(const artefact_process_params *p, const artefact_channe
 int artefact_hip_plan_set_precision(artefact_hip_plan *pl, uint32_t precision)
 {
     if (!pl) return ARTEFACT_E_ARGUMENT;
-    if (precision != ARTEFACT_PRECISION_F32 && precision != ARTEFACT_PRECISION_BF16X3) return ARTEFACT_E_ARGUMENT;
+    if (precision != ARTEFACT_PRECISION_F32 && precision != ARTEFACT_PRECISION_BF16X3 &&
+        precision != ARTEFACT_PRECISION_F32_FAST)
+        return ARTEFACT_E_ARGUMENT;
     if (precision == ARTEFACT_PRECISION_BF16X3)
         for (uint32_t c = 0; c < pl->C; c++)
             if (pl->ch[c].mode != PM_FULL) return ARTEFACT_E_GEOMETRY;

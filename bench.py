@@ -122,6 +122,50 @@ def copy_calibration(nbytes=256 << 20, reps=4):
     return {"bytes_read_per_copy": nbytes, "copy_GBps": round(2 * nbytes * reps / (ms.value * 1e-3) / 1e9, 1)}
 
 
+def fast_comparison(wl, args, exact_out, exact_objective, B):
+    """The same workload with ARTEFACT_PRECISION_F32_FAST: throughput, iteration time and its
+    roofline fraction, and the distance of its result from the bit-exact one."""
+    import numpy as np
+
+    import artefact_amd
+
+    W, H, iters = wl["max_w"], wl["max_h"], wl["iterations"]
+    plan = artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"], graph=True)
+    plan.set_precision("f32_fast")
+    tuning = plan.tune()
+    for _ in range(args.warmup):
+        plan.init()
+        plan.iterate()
+    plan.download()
+    plan.set_timing(args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.init()
+        plan.iterate()
+    plan.last_norms()
+    elapsed = time.perf_counter() - t0
+    loop_ms, launches = plan.timing()
+    t_iter_ms = loop_ms / (launches * iters)
+    plan.init()
+    plan.iterate()
+    objective, _ = plan.objective()
+    out = plan.download()
+    plan.close()
+    diff = np.concatenate([np.abs(a - b) for a, b in zip(out, exact_out)])
+    achieved = B / (t_iter_ms * 1e-3) / 1e9
+    return {
+        "precision": "f32_fast",
+        "value": round(W * H * iters * args.steps / elapsed / 1e6, 2),
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "iteration_ms": round(t_iter_ms, 5),
+        "roofline_frac": round(achieved / HBM_PEAK_GBS, 4),
+        "objective_rel_to_exact": float(f"{(objective - exact_objective) / exact_objective:.3g}"),
+        "max_abs_vs_exact": float(f"{diff.max():.3g}"),
+        "share_within_0.05_of_exact": float(f"{np.mean(diff <= 0.05):.6f}"),
+        "autotuned": tuning,
+    }
+
+
 def host_cpu():
     """(CPU model name, CPUs this process may run on)."""
     model = None
@@ -559,6 +603,9 @@ def main() -> int:
     ap.add_argument("--converged-iterations", type=int, default=300)
     ap.add_argument("--warm-sweep", type=int, nargs="*", default=[15, 25, 40, 60])
     ap.add_argument("--copy-sweep", action="store_true", help="also time 16/64/128 MiB copies")
+    ap.add_argument("--precision", default="f32", choices=["f32", "f32_fast"],
+                    help="arithmetic of the timed loop (default: the bit-exact f32 drop-in)")
+    ap.add_argument("--no-fast", action="store_true", help="skip the f32_fast comparison of the default line")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print per-kernel HBM bytes from two rocprofv3 --pmc CSVs and exit")
     args = ap.parse_args()
@@ -592,6 +639,7 @@ def main() -> int:
           file=sys.stderr, flush=True)
 
     plan = artefact_amd.Plan(coefs, wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"], graph=True)
+    plan.set_precision(args.precision)
     tuning = plan.tune()
     for _ in range(args.warmup):
         plan.init()
@@ -627,6 +675,9 @@ def main() -> int:
     out = plan.download()
     finite = all(bool(np.isfinite(p).all()) for p in out)
     plan.close()
+    fast = None
+    if args.precision == "f32" and not args.no_fast:
+        fast = fast_comparison(wl, args, out, objective, B)
     calib = copy_calibration()
     if calib is not None and args.copy_sweep:
         calib["copy_GBps_by_size_MiB"] = {mb: copy_calibration(mb << 20, reps=8)["copy_GBps"] for mb in (16, 64, 128)}
@@ -675,6 +726,10 @@ def main() -> int:
         },
         "finite_output": finite,
     }
+    if args.precision != "f32":
+        res["dtype"] = f"f32 ({args.precision}: hardware sqrt/reciprocal in the gradient sweep, not bit-exact)"
+    if fast is not None:
+        res["fast_f32"] = fast
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, args.cpu_iters)
     print(json.dumps(res), flush=True)

@@ -42,6 +42,7 @@ extern "C" {
 
 #define ARTEFACT_PRECISION_F32    0  /* the reference's f32 arithmetic, bit for bit (default) */
 #define ARTEFACT_PRECISION_BF16X3 1  /* projection DCT/IDCTs on BF16 MFMA, hi/lo split, f32 accumulation */
+#define ARTEFACT_PRECISION_F32_FAST 2 /* f32 gradient with hardware sqrt/reciprocal, plain f64 norm sum */
 
 /* One colour component: backend/artefact-lib/jpeg/mod.rs:8-27 (Coefficient).
  * The reference carries coefficients and the quantisation table as f32 holding
@@ -124,11 +125,14 @@ void *artefact_hip_plan_stream(artefact_hip_plan *plan);
 int artefact_hip_plan_set_graph(artefact_hip_plan *plan, int enable);
 void artefact_hip_plan_destroy(artefact_hip_plan *plan);
 
-/* Selects the projection's arithmetic: ARTEFACT_PRECISION_F32 (default, bit-identical to the
- * reference) or ARTEFACT_PRECISION_BF16X3 (the precision form: every 8x8 DCT and IDCT of the
- * projection as BF16 MFMA contractions with hi/lo operands and f32 accumulation; the stencil
- * and descent stay f32).  BF16X3 needs every channel at full grid resolution (4:4:4 or one
- * channel), else ARTEFACT_E_GEOMETRY.  No reference counterpart. */
+/* Selects the loop's arithmetic: ARTEFACT_PRECISION_F32 (default, bit-identical to the
+ * reference's f32 operations), ARTEFACT_PRECISION_BF16X3 (the precision form: every 8x8 DCT
+ * and IDCT of the projection as BF16 MFMA contractions with hi/lo operands and f32
+ * accumulation; the stencil and descent stay f32) or ARTEFACT_PRECISION_F32_FAST (the
+ * gradient sweep with v_sqrt_f32 / v_rcp_f32 and (alpha / n) * term in place of the correctly
+ * rounded (alpha * term) / n, and a plain f64 sum for the norm; projection unchanged; within a
+ * few ulp per term, not bit-identical).  BF16X3 needs every channel at full grid resolution
+ * (4:4:4 or one channel), else ARTEFACT_E_GEOMETRY.  No reference counterpart. */
 int artefact_hip_plan_set_precision(artefact_hip_plan *plan, uint32_t precision);
 
 /* Autotuner result: the k_gradient strip height (output rows per wave) and the projection
