@@ -2512,6 +2512,10 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     if (const char *e = getenv("ARTEFACT_FUSE_NORM")) pl->fuse_norm = atoi(e) != 0;
     if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
         if (atoi(e)) pl->layout = LAYOUT_GENERIC;
+    // Lets callers of the one-shot entry points (artefact_hip_compute / _process, batches) opt
+    // into the fast gradient arithmetic without an API change.
+    if (const char *e = getenv("ARTEFACT_PRECISION"))
+        if (strcmp(e, "f32_fast") == 0) pl->precision = ARTEFACT_PRECISION_F32_FAST;
     ok = ok && hipMemcpyAsync(pl->quant, quant.data(), quant.size() * sizeof(float), hipMemcpyHostToDevice,
                               pl->stream) == hipSuccess;
     ok = ok && hipMemcpyAsync(pl->rqq, rqq.data(), rqq.size() * sizeof(double), hipMemcpyHostToDevice,

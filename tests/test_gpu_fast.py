@@ -112,3 +112,20 @@ def test_back_to_exact_is_bit_exact():
     ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], 7, W, H, norm_mode="f64")
     for a, b in zip(out, ref):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_environment_selects_fast_for_one_shot_calls(monkeypatch):
+    """ARTEFACT_PRECISION=f32_fast: artefact_hip_compute runs the fast loop (the same bits as a
+    plan switched to it by hand)."""
+    wl = synth.workload("tgv_420_96x64", seed=24)
+    W, H = wl["max_w"], wl["max_h"]
+    by_hand, _ = solve(wl, 9, "f32_fast")
+    monkeypatch.setenv("ARTEFACT_PRECISION", "f32_fast")
+    env = artefact_amd.compute(3, wl["coefs"], wl["weight"], wl["pweight"], 9, W, H, W * H)
+    for a, b in zip(env, by_hand):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    monkeypatch.delenv("ARTEFACT_PRECISION")
+    exact = artefact_amd.compute(3, wl["coefs"], wl["weight"], wl["pweight"], 9, W, H, W * H)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], 9, W, H, norm_mode="f64")
+    for a, b in zip(exact, ref):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
