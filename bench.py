@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 ITERATION_KERNELS = ("k_gradient", "k_finalize", "k_project")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tgv_420_2048x1536.json")
+TRAFFIC_FILE_FAST = os.path.join(ROOT, "profiles", "traffic_tgv_420_2048x1536_fast.json")
 
 
 def summarize_pmc(fetch_csv, write_csv, copy_bytes=256 << 20):
@@ -682,8 +683,9 @@ def main() -> int:
     if calib is not None and args.copy_sweep:
         calib["copy_GBps_by_size_MiB"] = {mb: copy_calibration(mb << 20, reps=8)["copy_GBps"] for mb in (16, 64, 128)}
     traffic = None
-    if os.path.exists(TRAFFIC_FILE) and args.workload == "tgv_420_2048":
-        with open(TRAFFIC_FILE) as fh:
+    tfile = TRAFFIC_FILE_FAST if args.precision == "f32_fast" else TRAFFIC_FILE
+    if os.path.exists(tfile) and args.workload == "tgv_420_2048":
+        with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_iteration")
 
     res = {
