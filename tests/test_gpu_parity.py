@@ -341,3 +341,28 @@ def test_golden_fixtures_bit_exact():
                                    case["iterations"], W, H, W * H, mode=case["mode"])
         for c, (a, b) in enumerate(zip(gpu, case["planes"])):
             assert bits_equal(a, b), f"{case['name']} channel {c}: max abs diff {np.abs(a - b).max()}"
+
+
+def test_concurrent_compute_calls_match_sequential():
+    """Separate mode in the reference calls compute(1, ...) for each channel concurrently from
+    rayon workers (lib.rs:140-149).  Four host threads, each with its own one-channel image,
+    call the C ABI at once; every result must equal the oracle's bits, as sequential calls
+    do."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    jobs = []
+    for seed in range(4):
+        wl = synth.workload("tgv_420_96x64", seed=30 + seed)
+        k = wl["coefs"][seed % 3]  # luma or an upsampled chroma component, on the image's grid
+        jobs.append((k, wl["max_w"], wl["max_h"]))
+
+    def run(job):
+        k, W, H = job
+        return artefact_amd.compute(1, [k], 0.3, [0.001] * 3, 12, W, H, W * H)
+
+    print("case concurrent compute x4", flush=True)
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        results = list(pool.map(run, jobs * 2))
+    for (k, W, H), got in zip(jobs * 2, results):
+        ref = oracle.compute([k], 0.3, [0.001] * 3, 12, W, H, norm_mode="f64")
+        assert bits_equal(got[0], ref[0])
