@@ -615,7 +615,9 @@ struct NormShared {
 // loads in flight; a projection kernel: 4, to keep its register budget).  The result is the
 // same for every BATCH: thread t merges partials t, t + NF, t + 2 NF, ... in that order
 // (a zero pair past the last one changes nothing).
-template <int C, int BATCH>
+// PLAIN (the fast f32 mode, whose partials carry no low part): the same tree with plain f64
+// additions, a much shorter dependent chain.
+template <int C, int BATCH, bool PLAIN = false>
 __device__ __forceinline__ void reduce_norms(const double *partials, unsigned int nwg, bool writer, float *norms,
                                              double *rnorm, double *norm_sq, NormShared &out)
 {
@@ -641,14 +643,20 @@ __device__ __forceinline__ void reduce_norms(const double *partials, unsigned in
 #pragma unroll
             for (int u = 0; u < BATCH; u++)
 #pragma unroll
-                for (int c = 0; c < C; c++) s[c] = dd_merge(s[c], DD{v[u][2 * c], v[u][2 * c + 1]});
+                for (int c = 0; c < C; c++) {
+                    if constexpr (PLAIN) s[c].hi += v[u][2 * c];
+                    else s[c] = dd_merge(s[c], DD{v[u][2 * c], v[u][2 * c + 1]});
+                }
         }
     }
 #pragma unroll
     for (int c = 0; c < C; c++) {
         DD v = s[c];
 #pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) v = dd_merge(v, dd_shfl_xor(v, m));
+        for (int m = 32; m >= 1; m >>= 1) {
+            if constexpr (PLAIN) v.hi += __shfl_xor(v.hi, m, 64);
+            else v = dd_merge(v, dd_shfl_xor(v, m));
+        }
         if (lane == 0) out.red[wave][c] = v;
     }
     __syncthreads();
@@ -656,7 +664,10 @@ __device__ __forceinline__ void reduce_norms(const double *partials, unsigned in
         const int c = tid;
         DD v = out.red[0][c];
 #pragma unroll
-        for (int w = 1; w < NF / 64; w++) v = dd_merge(v, out.red[w][c]);
+        for (int w = 1; w < NF / 64; w++) {
+            if constexpr (PLAIN) v.hi += out.red[w][c].hi;
+            else v = dd_merge(v, out.red[w][c]);
+        }
         const double sum = v.hi + v.lo;
         const float norm = (float)sqrt(sum);
         const double rn = norm != 0.0f ? 1.0 / (double)norm : 0.0;
@@ -672,12 +683,12 @@ __device__ __forceinline__ void reduce_norms(const double *partials, unsigned in
 }
 
 // The reduction as a kernel of its own (one workgroup), between k_gradient and the projection.
-template <int C>
+template <int C, bool PLAIN = false>
 __global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigned int nwg, float *norms,
                                                  double *rnorm, double *norm_sq)
 {
     __shared__ NormShared ns;
-    reduce_norms<C, 8>(partials, nwg, true, norms, rnorm, norm_sq, ns);
+    reduce_norms<C, 8, PLAIN>(partials, nwg, true, norms, rnorm, norm_sq, ns);
 }
 
 // A projection kernel's view of the norms: reduced by its own workgroup from the k_gradient
@@ -2041,9 +2052,14 @@ void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a, hipStream
         hipLaunchKernelGGL((k_gradient<C, TGV, true>), pl->grid_g, dim3(NT), 0, s, a);
     else
         hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, s, a);
-    if (!a.fuse_norm)
-        hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials, a.nwg_grad,
-                           pl->norms, pl->rnorm, pl->norm_sq);
+    if (!a.fuse_norm) {
+        if (pl->precision == ARTEFACT_PRECISION_F32_FAST)
+            hipLaunchKernelGGL((k_finalize<C, true>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials,
+                               a.nwg_grad, pl->norms, pl->rnorm, pl->norm_sq);
+        else
+            hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials, a.nwg_grad,
+                               pl->norms, pl->rnorm, pl->norm_sq);
+    }
 }
 
 // Gradient sweep + norm: k_gradient then k_finalize (unless the projection that follows
