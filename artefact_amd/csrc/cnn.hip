@@ -16,6 +16,11 @@
 // MFMA with the pixels as the contraction dimension, per-workgroup partials reduced in a
 // fixed order.  Operand lane layouts (16x16x16, 64 lanes): A[m][k] at lane m + 16*(k/4) item
 // k%4; B[k][n] at lane n + 16*(k/4) item k%4; C[m][n] at lane n + 16*(m/4) item m%4.
+//
+// Inference (k_cnn_stream) runs all five layers in one kernel that streams down 64-column
+// bands, with activations kept in LDS rows and v_mfma_f32_16x16x32_bf16 (K = 32: lane l holds
+// k = 8*(l/16) .. +7 of A row / B column l%16; C as above).  The layered kernels stay for
+// training and as the ARTEFACT_CNN_LAYERED=1 inference path.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
