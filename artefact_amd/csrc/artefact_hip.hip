@@ -1944,6 +1944,7 @@ struct artefact_hip_plan {
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
     ChanDev ch[3];
+    void *arena = nullptr;               // the one device allocation every buffer below lives in
     float *buf[2] = {nullptr, nullptr};  // each C*N: x and x_prev, swapped every iteration
     float *g = nullptr;                  // C*N
     float *ybuf = nullptr;               // C*N: extrapolated image of the current iteration
@@ -2003,10 +2004,7 @@ void free_plan(artefact_hip_plan *pl)
     DeviceGuard guard(pl->device);
     if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
     for (hipEvent_t e : pl->timing_events) (void)hipEventDestroy(e);
-    void *bufs[] = {pl->buf[0], pl->buf[1], pl->ybuf, pl->g,        pl->cos,     pl->prob,  pl->coef, pl->quant,
-                    pl->rqq,    pl->partials, pl->norms, pl->rnorm, pl->norm_sq};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
+    if (pl->arena) (void)hipFree(pl->arena);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);
     delete pl;
 }
@@ -2450,19 +2448,36 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     // change the height without reallocating.
     const size_t nwg = (size_t)pl->grid_g.x * ((pl->H + WAVES * 2 - 1) / (WAVES * 2));
     bool ok = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) == hipSuccess;
-    ok = ok && hipMalloc(&pl->buf[0], pl->C * pl->N * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->buf[1], pl->C * pl->N * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->g, pl->C * pl->N * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->ybuf, pl->C * pl->N * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->cos, mtot * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->prob, mtot * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->coef, mtot * sizeof(int16_t)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->quant, 3 * 64 * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->rqq, 3 * 64 * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->partials, nwg * 6 * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->norms, 3 * sizeof(float)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->rnorm, 3 * sizeof(double)) == hipSuccess;
-    ok = ok && hipMalloc(&pl->norm_sq, 3 * sizeof(double)) == hipSuccess;
+    // One device allocation holds every buffer, each at a 256-byte boundary: a one-shot call
+    // (artefact_hip_compute / _process) pays one hipMalloc and one hipFree, not 13 of each.
+    const size_t cn = (size_t)pl->C * pl->N;
+    size_t arena_bytes = 0;
+    auto take = [&arena_bytes](size_t bytes) {
+        const size_t at = arena_bytes;
+        arena_bytes += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t at_x = take(cn * 4), at_xp = take(cn * 4), at_g = take(cn * 4), at_y = take(cn * 4);
+    const size_t at_cos = take(mtot * 4), at_prob = take(mtot * 4), at_coef = take(mtot * 2);
+    const size_t at_quant = take(3 * 64 * 4), at_rqq = take(3 * 64 * 8), at_part = take(nwg * 6 * 8);
+    const size_t at_norms = take(3 * 4), at_rnorm = take(3 * 8), at_nsq = take(3 * 8);
+    ok = ok && hipMalloc(&pl->arena, arena_bytes) == hipSuccess;
+    if (ok) {
+        char *base = static_cast<char *>(pl->arena);
+        pl->buf[0] = reinterpret_cast<float *>(base + at_x);
+        pl->buf[1] = reinterpret_cast<float *>(base + at_xp);
+        pl->g = reinterpret_cast<float *>(base + at_g);
+        pl->ybuf = reinterpret_cast<float *>(base + at_y);
+        pl->cos = reinterpret_cast<float *>(base + at_cos);
+        pl->prob = reinterpret_cast<float *>(base + at_prob);
+        pl->coef = reinterpret_cast<int16_t *>(base + at_coef);
+        pl->quant = reinterpret_cast<float *>(base + at_quant);
+        pl->rqq = reinterpret_cast<double *>(base + at_rqq);
+        pl->partials = reinterpret_cast<double *>(base + at_part);
+        pl->norms = reinterpret_cast<float *>(base + at_norms);
+        pl->rnorm = reinterpret_cast<double *>(base + at_rnorm);
+        pl->norm_sq = reinterpret_cast<double *>(base + at_nsq);
+    }
     if (!ok) {
         free_plan(pl);
         return ARTEFACT_E_HIP;
