@@ -219,6 +219,7 @@ def bench_precision(args) -> int:
     for prec in ("f32", "bf16x3"):
         plan = artefact_amd.Plan(coefs, wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"], graph=True)
         plan.set_precision(prec)
+        tuned = plan.tune()
         for _ in range(args.warmup):
             plan.init()
             plan.iterate()
@@ -242,7 +243,7 @@ def bench_precision(args) -> int:
         res_by[prec] = {"ms_per_step": round(ms, 4), "iteration_ms": round(loop_ms / (launches * iters), 5),
                         "eager_breakdown_ms": {"k_gradient_and_finalize": round(prof["k_gradient_ms"], 5),
                                                "k_project": round(prof["k_project_ms"], 5)},
-                        "objective": obj}
+                        "objective": obj, "autotuned": tuned}
         print(f"bench: {prec} {ms:.3f} ms/solve", file=sys.stderr, flush=True)
     errors = None
     if not args.no_error:
