@@ -841,6 +841,9 @@ int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const floa
                        uint32_t h, uint32_t w, float *const *out, hipStream_t s)
 {
     if (!net || !planes || !qsteps || !out || !nplanes || nplanes > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
+    // A training step returns with its Adam update and weight repacking still queued on the
+    // net's stream; the kernels below read those weights on stream s.
+    if (hipStreamSynchronize(net->stream) != hipSuccess) return ARTEFACT_E_HIP;
     bool ok = true;
     if (!layered_requested()) {
         if (net->ws_stale) {
