@@ -23,6 +23,7 @@
 // training and as the ARTEFACT_CNN_LAYERED=1 inference path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -644,10 +645,21 @@ __device__ __forceinline__ void stream_last(uint8_t *lds, const float *__restric
 // Grid: (bands of SB columns, segments of RH rows); RH a multiple of 3.  Eight waves: wave 0
 // layer 1, waves 1-6 layers 2-4 (two channel halves each), wave 7 layer 5.
 constexpr int SCT = 512;
-__global__ __launch_bounds__(SCT, 4) void k_cnn_stream(const float *__restrict__ x0, const u32x4 *__restrict__ wts,
-                                                       const float *__restrict__ params, float *__restrict__ out,
-                                                       int H, int W, int RH, float qstep)
+// Up to SPLANES planes per launch (blockIdx.z), so one plane's last workgroups overlap the next
+// plane's first.
+constexpr int SPLANES = 4;
+struct StreamPlanes {
+    const float *x0[SPLANES];
+    float *out[SPLANES];
+    float qstep[SPLANES];
+};
+
+__global__ __launch_bounds__(SCT, 4) void k_cnn_stream(const StreamPlanes pp, const u32x4 *__restrict__ wts,
+                                                       const float *__restrict__ params, int H, int W, int RH)
 {
+    const float *__restrict__ x0 = pp.x0[blockIdx.z];
+    float *__restrict__ out = pp.out[blockIdx.z];
+    const float qstep = pp.qstep[blockIdx.z];
     __shared__ __attribute__((aligned(16))) uint8_t lds[SLDS];
     for (int i = threadIdx.x; i < SLDS / 16; i += SCT) reinterpret_cast<u32x4 *>(lds)[i] = u32x4{0u, 0u, 0u, 0u};
     __syncthreads();
@@ -858,9 +870,16 @@ int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const floa
         int rh = ((int)h + nby - 1) / nby;
         rh = std::max(12, (rh + 2) / 3 * 3);
         nby = ((int)h + rh - 1) / rh;
-        for (uint32_t i = 0; i < nplanes && ok; i++) {
-            hipLaunchKernelGGL(k_cnn_stream, dim3(nbx, nby), dim3(SCT), 0, s, planes[i], (const u32x4 *)net->ws,
-                               (const float *)net->params, out[i], (int)h, (int)w, rh, qsteps[i]);
+        for (uint32_t i0 = 0; i0 < nplanes && ok; i0 += SPLANES) {
+            StreamPlanes pp{};
+            const uint32_t n = std::min<uint32_t>(SPLANES, nplanes - i0);
+            for (uint32_t k = 0; k < n; k++) {
+                pp.x0[k] = planes[i0 + k];
+                pp.out[k] = out[i0 + k];
+                pp.qstep[k] = qsteps[i0 + k];
+            }
+            hipLaunchKernelGGL(k_cnn_stream, dim3(nbx, nby, n), dim3(SCT), 0, s, pp, (const u32x4 *)net->ws,
+                               (const float *)net->params, (int)h, (int)w, rh);
             ok = hipGetLastError() == hipSuccess;
         }
         return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
