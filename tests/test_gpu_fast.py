@@ -58,7 +58,8 @@ def assert_within_reference_spread(wl, iters, fast, f_fast):
     assert rel <= max(1e-4, 2.0 * obj_spread), (rel, obj_spread)
 
 
-@pytest.mark.parametrize("name", ["tgv_420_208x144", "tgv_444_72x40", "tv_luma_64x64"])
+@pytest.mark.parametrize("name", ["tgv_420_208x144", "tgv_444_72x40", "tv_luma_64x64", "tgv_422_80x40",
+                                  "tgv_440_48x64"])
 def test_fast_within_the_reference_spread(name):
     wl = synth.workload(name, seed=21)
     print(f"case fast {name} iters=30", flush=True)
