@@ -87,14 +87,15 @@ def test_adam_lowers_the_loss():
     assert losses[-1] < 0.25 * losses[0], (losses[0], losses[-1])
 
 
-@pytest.mark.parametrize("H,W", [(24, 48), (70, 100), (200, 136)])
+@pytest.mark.parametrize("H,W", [(24, 48), (70, 100), (200, 136), (9, 17), (8, 8)])
 def test_streaming_kernel_matches_layered(H, W, monkeypatch):
     """Inference runs all five layers in one streaming kernel (v_mfma_f32_16x16x32_bf16, each
     tap over all 32 channels at once).  The layered kernels (16x16x16, two channel halves per
     tap) stay behind ARTEFACT_CNN_LAYERED=1.  Both round to bf16 between layers after
     different MFMA groupings of the same sums, so they agree to bf16 precision, not bit for
     bit: the residuals within 1 % (relative L2) and, at every pixel, within 2 % of the
-    residual's range.  The sizes give partial bands and tiles and several row segments."""
+    residual's range.  The sizes give partial bands and tiles, several row segments, and
+    planes smaller than one segment and one tile."""
     flat, _, _ = params_with_bias(11)
     x0, q, _ = batch(12, B=2, H=H, W=W)
     print(f"case cnn streaming vs layered {H}x{W}", flush=True)
