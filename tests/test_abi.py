@@ -76,3 +76,18 @@ def test_coefficient_conversion_is_exact_or_refused():
     assert c.dct_coefs.dtype == np.int16 and c.quant_table.dtype == np.uint16
     with pytest.raises(ValueError):
         Coefficient(8, 8, 1, 1, 1, 1, np.full(64, 0.5, np.float32), np.ones(64))
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    """No CPU fallback: without the HIP library every entry point raises instead of computing
+    something else."""
+    from artefact_amd import synth
+
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libartefact_hip.so")
+    monkeypatch.setattr(_lib, "_lib", None)
+    wl = synth.workload("tgv_420_16x16", seed=1)
+    W, H = wl["max_w"], wl["max_h"]
+    with pytest.raises(RuntimeError, match="missing"):
+        artefact_amd.compute(3, wl["coefs"], 0.3, [0.001] * 3, 2, W, H, W * H)
+    with pytest.raises(RuntimeError, match="missing"):
+        artefact_amd.Plan(wl["coefs"], 0.3, [0.001] * 3, 2, W, H)
