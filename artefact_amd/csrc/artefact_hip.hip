@@ -2331,9 +2331,10 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
             int per_cu = 0, cus = 0;
             gradient_residency(pl->device, pl->C, pl->tgv, pl->precision == ARTEFACT_PRECISION_F32_FAST, per_cu,
                                cus);
-            // Shorter strips run several rounds at full occupancy: on large images, where the
-            // sweep waits on memory, that beats one round of few waves per SIMD (4096x3072
-            // 4:4:4: 337 us per iteration at RW = 18 against 351 us at the one-round 70).
+            // Shorter strips run several waves of workgroups at full occupancy: on large images,
+            // where the sweep waits on memory, that beats a single wave of workgroups with few
+            // waves per SIMD (4096x3072 4:4:4: 337 us per iteration at RW = 18 against 351 us
+            // at the single-wave 70).
             std::vector<int> cand{r0 + 1, r0 + 2, r0 - 1, r0 - 2, 2 * r0, r0 / 2, r0 / 3, r0 / 4};
             for (int k = per_cu - 1; k >= std::max(1, per_cu - 2); k--) {
                 const int rk = rows_for_workgroups(pl->W, pl->H, (uint64_t)k * cus);
@@ -2962,6 +2963,8 @@ int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
 {
     if (!pl || !net) return ARTEFACT_E_ARGUMENT;
     DeviceGuard guard(pl->device);
+    // The buffers are overwritten from here on: a failure below leaves no start image.
+    pl->initialized = false;
     launch_init(pl, pl->stream);
     const float *planes[3];
     float *pred[3];

@@ -1009,13 +1009,13 @@ int artefact_cnn_set_dataset(artefact_cnn *net, const float *x0, const float *qs
 int artefact_cnn_train_step(artefact_cnn *net, const uint32_t *indices, uint32_t batch, float lr, double *loss)
 {
     if (!net || !indices || !batch || batch > 64 || !net->ds_count) return ARTEFACT_E_ARGUMENT;
+    for (uint32_t i = 0; i < batch; i++)
+        if (indices[i] >= net->ds_count) return ARTEFACT_E_ARGUMENT;
     const int H = (int)net->ds_h, W = (int)net->ds_w;
     const size_t plane = (size_t)H * W;
     if (!ensure_workspace(net, plane * batch)) return ARTEFACT_E_HIP;
     bool ok = true;
-    std::vector<float> q(batch);
     for (uint32_t i = 0; i < batch && ok; i++) {
-        if (indices[i] >= net->ds_count) return ARTEFACT_E_ARGUMENT;
         ok = hipMemcpyAsync(net->x0 + i * plane, net->ds_x0 + indices[i] * plane, plane * 4,
                             hipMemcpyDeviceToDevice, net->stream) == hipSuccess;
         ok = ok && hipMemcpyAsync(net->target + i * plane, net->ds_target + indices[i] * plane, plane * 4,
@@ -1023,6 +1023,8 @@ int artefact_cnn_train_step(artefact_cnn *net, const uint32_t *indices, uint32_t
         ok = ok && hipMemcpyAsync(net->qstep + i, net->ds_q + indices[i], 4, hipMemcpyDeviceToDevice, net->stream) ==
                        hipSuccess;
     }
+    // A failed copy leaves the batch incomplete: no step is taken on it.
+    if (!ok) return ARTEFACT_E_HIP;
     hipLaunchKernelGGL(k_pack_input, dim3(blocks(plane * batch)), dim3(CT), 0, net->stream, (const float *)net->x0,
                        (const float *)net->qstep, plane, (int)batch, net->act[0], net->resid);
     forward(net, (int)batch, H, W, net->stream);

@@ -6,7 +6,7 @@ decoded start image of a synthetic crop (already inside the quantisation box) an
 TGV loop's output after ``target_iterations`` iterations.  Forward, backward and Adam run on
 the GPU.  ``warm_solve`` starts a plan from the projected prediction, runs its iterations and
 falls back to the plain loop when the warm result's objective is higher than the plain
-result's (the plain solver is the judge).  No reference counterpart.
+result's (the objective decides).  No reference counterpart.
 """
 from __future__ import annotations
 

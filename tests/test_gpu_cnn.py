@@ -87,6 +87,24 @@ def test_adam_lowers_the_loss():
     assert losses[-1] < 0.25 * losses[0], (losses[0], losses[-1])
 
 
+def test_out_of_range_index_takes_no_step():
+    """A batch with an index past the dataset is rejected before anything runs: the weights
+    stay as they were, and the next valid step still trains."""
+    from artefact_amd import _lib
+
+    x0, q, t = batch(3, B=2, H=16, W=32)
+    net = warmstart.WarmStartNet(warmstart.init_params(1))
+    net.set_dataset(x0, q, t)
+    before = net.params()
+    with pytest.raises(_lib.ArtefactError) as err:
+        net.train_step([0, 2], 1e-3)
+    assert err.value.code == -1
+    np.testing.assert_array_equal(net.params(), before)
+    net.train_step([0, 1], 1e-3)
+    assert not np.array_equal(net.params(), before)
+    net.close()
+
+
 @pytest.mark.parametrize("H,W", [(24, 48), (70, 100), (200, 136), (9, 17), (8, 8)])
 def test_streaming_kernel_matches_layered(H, W, monkeypatch):
     """Inference runs all five layers in one streaming kernel (v_mfma_f32_16x16x32_bf16, each

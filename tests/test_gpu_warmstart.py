@@ -3,8 +3,8 @@ on the GPU by `python -m artefact_amd.warmstart` on 20 000 solver-generated 256x
 
 What the network gains: on crops outside its training set, K plain iterations from the
 CNN's projected prediction end at a lower objective and closer to the converged (300
-iteration) result than K plain iterations from the decoded image.  The plain solver stays the
-judge: warm_solve falls back to the plain result when the warm one's objective is higher.
+iteration) result than K plain iterations from the decoded image.  The objective decides:
+warm_solve falls back to the plain result when the warm one's objective is higher.
 """
 import numpy as np
 import pytest
