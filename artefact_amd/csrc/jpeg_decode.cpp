@@ -48,6 +48,9 @@ struct Error {
     std::string msg;
 };
 
+// Largest accepted width and height, in pixels.
+constexpr int MAX_SIDE = 1 << 14;
+
 struct Huffman {
     bool set = false;
     int maxcode[18];  // largest code of each length, -1 if none (F.2.2.3)
@@ -258,6 +261,15 @@ struct Decoder {
         height = u16();
         width = u16();
         const int nc = byte();
+        // The size limits of the decoder's default options, checked before the zero size
+        // (zune-jpeg/src/headers.rs:216-230; zune.rs:12 builds the decoder with the defaults
+        // of zune-core 0.5.0-rc2, 1 << 14 per side).
+        if (width > MAX_SIDE)
+            throw Error{"image width " + std::to_string(width) + " greater than width limit " +
+                        std::to_string(MAX_SIDE)};
+        if (height > MAX_SIDE)
+            throw Error{"image height " + std::to_string(height) + " greater than height limit " +
+                        std::to_string(MAX_SIDE)};
         if (width == 0 || height == 0) throw Error{"zero image size"};
         if (nc < 1 || nc > 4) throw Error{"component count must be 1 to 4"};
         if (len != 8 + 3 * nc) throw Error{"bad SOF length"};

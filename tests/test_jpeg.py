@@ -150,6 +150,68 @@ def test_malformed_data_raises(data):
         artefact_amd.decode_jpeg(data)
 
 
+def _with_frame_size(data, w, h):
+    data = bytearray(data)
+    i = data.index(b"\xff\xc0")
+    data[i + 5:i + 7] = h.to_bytes(2, "big")
+    data[i + 7:i + 9] = w.to_bytes(2, "big")
+    return bytes(data)
+
+
+def test_frame_size_limit():
+    """zune-core's default decoder options (zune.rs:12) allow 1 << 14 pixels per side
+    (zune-jpeg/src/headers.rs:216-230).  A 16384-wide frame decodes (its missing entropy data
+    keeps zeros); one pixel more on either side is rejected before any plane is allocated."""
+    data = pil_jpeg(np.full((8, 16), 128, np.uint8), quality=90)
+    j = artefact_amd.decode_jpeg(_with_frame_size(data, 16384, 8))
+    assert (j.coefs[0].rounded_px_w, j.coefs[0].rounded_px_h) == (16384, 8)
+    with pytest.raises(artefact_amd.JpegDecodeError, match="width 16385 greater than width limit 16384"):
+        artefact_amd.decode_jpeg(_with_frame_size(data, 16385, 8))
+    with pytest.raises(artefact_amd.JpegDecodeError, match="height 65535 greater than height limit 16384"):
+        artefact_amd.decode_jpeg(_with_frame_size(data, 16, 65535))
+
+
+def test_mutated_files_decode_or_raise():
+    """Seeded byte flips, overwrites, insertions, deletions and truncations of valid files
+    (baseline and progressive, 4:2:0 / 4:4:4 / grey): each either decodes into planes of the
+    size it reports or raises JpegDecodeError."""
+    rng = np.random.default_rng(11)
+    rgb = np.stack(synth.synth_rgb(40, 24, 3), -1).astype(np.uint8)
+    seeds = [pil_jpeg(rgb, quality=q, subsampling=s, progressive=p)
+             for q, s, p in [(30, 2, False), (90, 0, True), (60, 2, True)]]
+    seeds.append(pil_jpeg(rgb[..., 0], quality=75, progressive=True))
+    decoded = rejected = 0
+    for _ in range(1500):
+        b = bytearray(seeds[rng.integers(len(seeds))])
+        for _ in range(rng.integers(1, 5)):
+            if not b:
+                break
+            at = int(rng.integers(len(b)))
+            op = rng.integers(6)
+            if op == 0:
+                b[at] ^= 1 << int(rng.integers(8))
+            elif op == 1:
+                b[at] = int(rng.integers(256))
+            elif op == 2:
+                b[at] = 0xFF
+            elif op == 3:
+                del b[at:]
+            elif op == 4:
+                b.insert(at, int(rng.integers(256)))
+            else:
+                del b[at:at + int(rng.integers(1, 16))]
+        try:
+            j = artefact_amd.decode_jpeg(bytes(b))
+        except artefact_amd.JpegDecodeError:
+            rejected += 1
+            continue
+        decoded += 1
+        for c in j.coefs:
+            assert c.dct_coefs.size == c.rounded_px_w * c.rounded_px_h
+            assert c.quant_table.size == 64
+    assert decoded > 100 and rejected > 100, (decoded, rejected)
+
+
 def test_unsupported_process_raises():
     data = bytearray(pil_jpeg(np.zeros((16, 16), np.uint8), quality=90))
     i = data.index(b"\xff\xc0")
