@@ -2450,23 +2450,30 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     pl->alpha_tgv = (pl->weight / sqrtf(2.0f)) / sqrtf(Cf);
     const float radius = sqrtf((float)pl->N) / 2.0f;
     pl->step = radius / sqrtf(1.0f + (float)pl->iterations);
-    pl->factors.resize(pl->iterations);
-    float term = 1.0f;
-    for (uint32_t i = 0; i < pl->iterations; i++) {
-        const float nt = (float)(((double)1.0f + (double)sqrtf(4.0f * (term * term) + 1.0f)) / 2.0);
-        pl->factors[i] = (term - 1.0f) / nt;
-        term = nt;
-    }
-
-    if (p->init_override) {
-        pl->init_override.resize(pl->C);
-        for (uint32_t c = 0; c < pl->C; c++) {
-            if (!p->init_override[c]) {
-                delete pl;
-                return ARTEFACT_E_ARGUMENT;
-            }
-            pl->init_override[c].assign(p->init_override[c], p->init_override[c] + pl->N);
+    // Host vectors sized by the caller's iteration count and grid: an allocation failure is
+    // an error code, never an exception through the C interface.
+    try {
+        pl->factors.resize(pl->iterations);
+        float term = 1.0f;
+        for (uint32_t i = 0; i < pl->iterations; i++) {
+            const float nt = (float)(((double)1.0f + (double)sqrtf(4.0f * (term * term) + 1.0f)) / 2.0);
+            pl->factors[i] = (term - 1.0f) / nt;
+            term = nt;
         }
+
+        if (p->init_override) {
+            pl->init_override.resize(pl->C);
+            for (uint32_t c = 0; c < pl->C; c++) {
+                if (!p->init_override[c]) {
+                    delete pl;
+                    return ARTEFACT_E_ARGUMENT;
+                }
+                pl->init_override[c].assign(p->init_override[c], p->init_override[c] + pl->N);
+            }
+        }
+    } catch (const std::bad_alloc &) {
+        delete pl;
+        return ARTEFACT_E_HIP;
     }
 
     DeviceGuard guard(pl->device);

@@ -150,6 +150,22 @@ def test_malformed_data_raises(data):
         artefact_amd.decode_jpeg(data)
 
 
+# The reference decoder's own invalid-input cases (zune-jpeg/tests/invalid_images.rs): each
+# must be rejected.  The messages are this decoder's own, so only the rejection is compared.
+@pytest.mark.parametrize("data", [
+    bytes([0xFF, 0xD8, 0xA4]),                                        # eof, :13-17
+    bytes([0xFF, 0xD8, 0xFF, 0x00, 0x00, 0x00]),                      # bad_ff_marker_size, :20-24
+    bytes([255, 216, 255, 218, 232, 197, 255]),                       # bad_number_of_scans, :27-35
+    bytes([255, 216, 255, 196, 0, 0]),                                # huffman_length_subtraction_overflow, :38-46
+    bytes([255, 216, 255, 218, 0, 8, 1, 0, 8, 1]),                    # index_oob, :49-53
+    bytes([255, 216, 255, 192, 255, 1, 8, 9, 119, 48, 255, 192]),     # mul_with_overflow, :56-66
+], ids=["eof", "bad_ff_marker_size", "bad_number_of_scans", "huffman_length_subtraction_overflow",
+        "index_oob", "mul_with_overflow"])
+def test_reference_invalid_images_rejected(data):
+    with pytest.raises(artefact_amd.JpegDecodeError):
+        artefact_amd.decode_jpeg(data)
+
+
 def _with_frame_size(data, w, h):
     data = bytearray(data)
     i = data.index(b"\xff\xc0")
