@@ -2225,8 +2225,9 @@ int run_colour(hipStream_t stream, const float *y, const float *cb, const float 
 }
 
 // ---- Autotuner: per geometry, the k_gradient strip height and the projection kernel ----
-// Candidates are timed on the plan itself (eager launches, HIP events around a few
-// iterations after a fresh init) and the winner is cached for the process.
+// Candidates are timed on the plan itself (a few iterations after a fresh init, replayed
+// from a graph: time_loop) and the winner is cached for the process, per device and geometry,
+// under g_tune_mu.
 struct TuneChoice {
     int rows, layout, lane, fuse;
     float ms;
