@@ -615,8 +615,8 @@ def main() -> int:
         print(json.dumps(summarize_pmc(*args.summarize_pmc), indent=1))
         return 0
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU form is a design note only)", file=sys.stderr)
+        print(f"bench.py: the reconstruction loop runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding images or plane bands across GPUs is not implemented)", file=sys.stderr)
         return 2
 
     import numpy as np
