@@ -299,7 +299,12 @@ using IC = std::integral_constant<int, N>;
 // v_sqrt_f32 for the joint norms, one v_rcp_f32 per norm and products (alpha / n) * term in
 // place of the correctly rounded (alpha * term) / n, and a plain f64 sum of g^2.  Each term is
 // then within a few ulp of the exact one instead of equal to it.
-template <int C, bool TGV, bool FAST = false>
+// WIDE: the same sweep and arithmetic for grids whose C planes reach 2^31 bytes, where the
+// 32-bit buffer offsets below (lane offset plus row and channel offset) could wrap.  Every
+// load and store then goes through a buffer resource over one row of one channel's plane,
+// based at that row's 64-bit address, with the lane offset alone: the same clamped loads,
+// zero loads and dropped stores, with offsets below one row's bytes.
+template <int C, bool TGV, bool FAST = false, bool WIDE = false>
 __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
 {
     __shared__ DD lred[WAVES][3][64];
@@ -354,20 +359,33 @@ __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
     // buffer range check (on the lane offset) turns into a dropped store or a zero load.
     const uint32_t arena = pbytes * C;
     const uint32_t vin = (uint32_t)min(max(x, 0), W - 1) * 4u;
-    const uint32_t vout = out_lane ? (uint32_t)x * 4u : arena;
+    const uint32_t rowb = (uint32_t)W * 4u;  // WIDE: bytes of a row, its out-of-range lane offset
+    const uint32_t vout = out_lane ? (uint32_t)x * 4u : (WIDE ? rowb : arena);
+    // WIDE: vpr is the offset in a component row, or the row's size (a zero load).
     uint32_t vpr[C];
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const int cx = x >> hsh[c];
-        vpr[c] = (out_lane && cx < crw[c]) ? poff[c] + (uint32_t)cx * 4u : a.prob_bytes;
+        if constexpr (WIDE)
+            vpr[c] = (out_lane && cx < crw[c]) ? (uint32_t)cx * 4u : (uint32_t)crw[c] * 4u;
+        else
+            vpr[c] = (out_lane && cx < crw[c]) ? poff[c] + (uint32_t)cx * 4u : a.prob_bytes;
     }
     auto rowload = [&](int t) {
         const uint32_t tr = (uint32_t)min(max(t, 0), H - 1);
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            const int so = (int)(tr * (uint32_t)W * 4u + (uint32_t)c * pbytes);
-            nx[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, (int)vin, so, 0));
-            nxp[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rxp, (int)vin, so, 0));
+            if constexpr (WIDE) {
+                const size_t ro = (size_t)tr * (size_t)W;
+                const __amdgpu_buffer_rsrc_t r1 = plane_rsrc(a.x[c] + ro, rowb);
+                const __amdgpu_buffer_rsrc_t r2 = plane_rsrc(a.xp[c] + ro, rowb);
+                nx[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r1, (int)vin, 0, 0));
+                nxp[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r2, (int)vin, 0, 0));
+            } else {
+                const int so = (int)(tr * (uint32_t)W * 4u + (uint32_t)c * pbytes);
+                nx[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rx, (int)vin, so, 0));
+                nxp[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rxp, (int)vin, so, 0));
+            }
         }
     };
     // Prob term of output row q, one step ahead of its use.  A row outside the component moves
@@ -379,9 +397,17 @@ __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
             const int cy = q >> vsh[c];
             const bool cin = cy < crh[c];
             // The row offset is clamped rather than selected, so no branch splits the step.
-            const int so = min(cy, crh[c] - 1) * crw[c] * 4;
-            dst[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                   rpr, (int)(cin ? vpr[c] : a.prob_bytes), so, 0));
+            if constexpr (WIDE) {
+                const uint32_t cb = (uint32_t)crw[c] * 4u;
+                const __amdgpu_buffer_rsrc_t rp =
+                    plane_rsrc(a.ch[c].prob + (size_t)min(cy, crh[c] - 1) * (size_t)crw[c], cb);
+                dst[c] = __builtin_bit_cast(float,
+                                            __builtin_amdgcn_raw_buffer_load_b32(rp, (int)(cin ? vpr[c] : cb), 0, 0));
+            } else {
+                const int so = min(cy, crh[c] - 1) * crw[c] * 4;
+                dst[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                       rpr, (int)(cin ? vpr[c] : a.prob_bytes), so, 0));
+            }
         }
     };
 
@@ -525,9 +551,22 @@ __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
                             gv += from_left(tgd[c]);
                             gv += tgy[c];
                         }
-                        const int so = (int)(qoff + (uint32_t)c * pbytes);
-                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), rg, (int)vst, so, 0);
-                        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yq[c]), ry, (int)vst, so, 0);
+                        if constexpr (WIDE) {
+                            // A row outside the image keeps row 0's base with the out-of-range offset.
+                            const size_t ro = qin ? (size_t)q * (size_t)W : 0;
+                            const uint32_t vw = qin ? vout : rowb;
+                            const __amdgpu_buffer_rsrc_t r1 = plane_rsrc(a.g[c] + ro, rowb);
+                            const __amdgpu_buffer_rsrc_t r2 = plane_rsrc(a.y[c] + ro, rowb);
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), r1, (int)vw, 0, 0);
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yq[c]), r2, (int)vw, 0,
+                                                                  0);
+                        } else {
+                            const int so = (int)(qoff + (uint32_t)c * pbytes);
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), rg, (int)vst, so,
+                                                                  0);
+                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yq[c]), ry, (int)vst,
+                                                                  so, 0);
+                        }
                         const float ga = acc_on ? gv : 0.0f;
                         if constexpr (FAST) acc[c].hi = __builtin_fma((double)ga, (double)ga, acc[c].hi);
                         else dd_add(acc[c], (double)ga * (double)ga);
@@ -1962,6 +2001,9 @@ struct artefact_hip_plan {
     int cur = 0;            // buf[cur] holds x
     bool initialized = false;
     bool use_graph = true;
+    // k_gradient with 64-bit addressing: the C planes of the grid reach 2^31 bytes (or
+    // ARTEFACT_WIDE_ADDRESSING=1).
+    bool wide = false;
     hipGraphExec_t graph = nullptr;
     // Optional event pairs around iterate()'s launches (artefact_hip_plan_set_timing).
     std::vector<hipEvent_t> timing_events;
@@ -1978,9 +2020,10 @@ int check_inputs(const artefact_params *p, const artefact_channel *ch)
     if (p->max_w < 8 || p->max_h < 8 || p->max_w % 8 || p->max_h % 8) return ARTEFACT_E_GEOMETRY;
     // int16 coefficients, u16 tables and a bounded grid keep every iterate far below
     // 2^60 in magnitude, so every quotient the kernels form is finite.
+    // The cap also keeps every element index within a plane below 2^29 and every byte offset
+    // within one plane below 2^31 (grids whose C planes together reach 2^31 bytes use the
+    // 64-bit-addressed gradient sweep).
     if ((uint64_t)p->max_w * p->max_h > (1ull << 29)) return ARTEFACT_E_GEOMETRY;
-    // The C planes of one arena are addressed with 32-bit byte offsets below 2^31.
-    if ((uint64_t)p->nchannel * p->max_w * p->max_h * 4 >= (1ull << 31)) return ARTEFACT_E_GEOMETRY;
     if (!(p->weight >= 0.0f && p->weight < 1e6f)) return ARTEFACT_E_ARGUMENT;
     for (int c = 0; c < 3; c++)
         if (!(p->pweight[c] >= 0.0f && p->pweight[c] < 1e6f)) return ARTEFACT_E_ARGUMENT;
@@ -2046,7 +2089,12 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
 template <int C, bool TGV>
 void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
-    if (pl->precision == ARTEFACT_PRECISION_F32_FAST)
+    const bool fast = pl->precision == ARTEFACT_PRECISION_F32_FAST;
+    if (pl->wide && fast)
+        hipLaunchKernelGGL((k_gradient<C, TGV, true, true>), pl->grid_g, dim3(NT), 0, s, a);
+    else if (pl->wide)
+        hipLaunchKernelGGL((k_gradient<C, TGV, false, true>), pl->grid_g, dim3(NT), 0, s, a);
+    else if (fast)
         hipLaunchKernelGGL((k_gradient<C, TGV, true>), pl->grid_g, dim3(NT), 0, s, a);
     else
         hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, s, a);
@@ -2153,9 +2201,25 @@ void launch_init(artefact_hip_plan *pl, hipStream_t s)
     }
 }
 
-// The k_gradient instance for (C, TGV, FAST), for occupancy queries.
-const void *gradient_kernel(uint32_t C, bool tgv, bool fast)
+// The k_gradient instance for (C, TGV, FAST, WIDE), for occupancy queries.
+const void *gradient_kernel(uint32_t C, bool tgv, bool fast, bool wide)
 {
+    if (wide) {
+        switch (C * 4 + (tgv ? 2 : 0) + (fast ? 1 : 0)) {
+        case 4: return (const void *)k_gradient<1, false, false, true>;
+        case 5: return (const void *)k_gradient<1, false, true, true>;
+        case 6: return (const void *)k_gradient<1, true, false, true>;
+        case 7: return (const void *)k_gradient<1, true, true, true>;
+        case 8: return (const void *)k_gradient<2, false, false, true>;
+        case 9: return (const void *)k_gradient<2, false, true, true>;
+        case 10: return (const void *)k_gradient<2, true, false, true>;
+        case 11: return (const void *)k_gradient<2, true, true, true>;
+        case 12: return (const void *)k_gradient<3, false, false, true>;
+        case 13: return (const void *)k_gradient<3, false, true, true>;
+        case 14: return (const void *)k_gradient<3, true, false, true>;
+        default: return (const void *)k_gradient<3, true, true, true>;
+        }
+    }
     switch (C * 4 + (tgv ? 2 : 0) + (fast ? 1 : 0)) {
     case 4: return (const void *)k_gradient<1, false>;
     case 5: return (const void *)k_gradient<1, false, true>;
@@ -2178,9 +2242,10 @@ const void *gradient_kernel(uint32_t C, bool tgv, bool fast)
 // workgroup, stacked) fits the device's resident workgroups at once.  An override
 // (ARTEFACT_KG_ROWS) is honoured.
 // Resident k_gradient workgroups per CU and the CU count (4 and 256 if the queries fail).
-void gradient_residency(int device, uint32_t C, bool tgv, bool fast, int &per_cu, int &cus)
+void gradient_residency(int device, uint32_t C, bool tgv, bool fast, bool wide, int &per_cu, int &cus)
 {
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gradient_kernel(C, tgv, fast), NT, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gradient_kernel(C, tgv, fast, wide), NT, 0) !=
+            hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || per_cu < 1 ||
         cus < 1) {
         per_cu = 4;
@@ -2198,14 +2263,14 @@ int rows_for_workgroups(uint32_t W, uint32_t H, uint64_t wgs)
     return (int)std::max(H, 1u);
 }
 
-int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv, bool fast)
+int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv, bool fast, bool wide)
 {
     if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
         const int r = atoi(e);
         if (r >= 1 && r <= 4096) return r;
     }
     int per_cu = 0, cus = 0;
-    gradient_residency(device, C, tgv, fast, per_cu, cus);
+    gradient_residency(device, C, tgv, fast, wide, per_cu, cus);
     return rows_for_workgroups(W, H, (uint64_t)per_cu * cus);
 }
 
@@ -2326,12 +2391,12 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
         const int r0 = pl->rows;
         std::vector<int> rows{r0};
         if (!getenv("ARTEFACT_KG_ROWS")) {
-            // Around the one-round height, and the heights that give every CU exactly
+            // Around the single-wave height, and the heights that give every CU exactly
             // per_cu - 1 or per_cu - 2 workgroups: the sweep is VALU-bound per SIMD, so its time
             // follows the most loaded CU's (workgroups x strip steps).
             int per_cu = 0, cus = 0;
-            gradient_residency(pl->device, pl->C, pl->tgv, pl->precision == ARTEFACT_PRECISION_F32_FAST, per_cu,
-                               cus);
+            gradient_residency(pl->device, pl->C, pl->tgv, pl->precision == ARTEFACT_PRECISION_F32_FAST, pl->wide,
+                               per_cu, cus);
             // Shorter strips run several waves of workgroups at full occupancy: on large images,
             // where the sweep waits on memory, that beats a single wave of workgroups with few
             // waves per SIMD (4096x3072 4:4:4: 337 us per iteration at RW = 18 against 351 us
@@ -2483,7 +2548,9 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         pl->M[c] = (size_t)chs[c].rounded_w * chs[c].rounded_h;
         mtot += pl->M[c];
     }
-    pl->rows = pick_rows(pl->device, pl->W, pl->H, pl->C, pl->tgv, false);
+    const char *wide_env = getenv("ARTEFACT_WIDE_ADDRESSING");
+    pl->wide = (uint64_t)pl->C * pl->N * 4 >= (1ull << 31) || (wide_env && wide_env[0] == '1');
+    pl->rows = pick_rows(pl->device, pl->W, pl->H, pl->C, pl->tgv, false, pl->wide);
     pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * pl->rows - 1) / (WAVES * pl->rows));
     pl->grid_p = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
     // Partials for the largest grid any strip height (>= 2) can give, so that the tuner can

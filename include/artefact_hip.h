@@ -29,7 +29,7 @@ extern "C" {
 /* Return codes. */
 #define ARTEFACT_OK              0
 #define ARTEFACT_E_ARGUMENT     -1  /* null pointer, nchannel not in 1..3, bad mode */
-#define ARTEFACT_E_GEOMETRY     -2  /* plane sizes not multiples of 8, inconsistent block counts, samp factor not 1/2, component larger than the grid, nchannel*max_w*max_h*4 >= 2^31 */
+#define ARTEFACT_E_GEOMETRY     -2  /* plane sizes not multiples of 8, inconsistent block counts, samp factor not 1/2, component larger than the grid, max_w*max_h > 2^29 */
 #define ARTEFACT_E_QUANT        -3  /* a quantisation table entry is 0 (the reference divides by q^2, compute_step_prob.rs:29) */
 #define ARTEFACT_E_HIP          -4  /* HIP runtime failure (allocation, launch, copy) */
 #define ARTEFACT_E_STATE        -5  /* plan used out of order (iterate before init) */

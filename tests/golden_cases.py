@@ -34,3 +34,35 @@ def loop_cases() -> list[dict]:
 def dct_blocks() -> dict:
     with np.load(os.path.join(GOLDEN, "dct_blocks.npz"), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}
+
+
+# The largest grid the decoder hands over: 16384 x 16384 4:2:0 (zune-jpeg/src/headers.rs:216-230).
+# Its three planes take 3 GiB, past the 2^31 bytes up to which one 32-bit buffer offset
+# addresses them all.
+MAX_GRID = {"W": 16384, "H": 16384, "seed": 20240611, "quality": 75, "weight": 0.3, "pweight": [0.001] * 3,
+            "iterations": 2}
+
+
+def random_420(W: int, H: int, seed: int, quality: int) -> list:
+    """Seeded 4:2:0 coefficients: DC in [-60, 60], the first nine AC terms in [-3, 3], the rest 0."""
+    from artefact_amd import synth
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for c, (w, h, s) in enumerate([(W, H, 1), (W // 2, H // 2, 2), (W // 2, H // 2, 2)]):
+        nb = (w // 8) * (h // 8)
+        k = np.zeros((nb, 64), np.int16)
+        k[:, 0] = rng.integers(-60, 61, nb, dtype=np.int16)
+        k[:, 1:10] = rng.integers(-3, 4, (nb, 9), dtype=np.int16)
+        out.append(Coefficient(w, h, w // 8, h // 8, s, s, k.reshape(-1), synth.quant_table(quality, c > 0)))
+    return out
+
+
+def max_grid_case() -> dict:
+    """MAX_GRID with the SHA-256 of each output plane (f32 bytes) from the oracle."""
+    import json
+
+    with open(os.path.join(GOLDEN, "max_grid_420.json")) as f:
+        fx = json.load(f)
+    assert {k: fx[k] for k in MAX_GRID} == MAX_GRID, "fixture written for other parameters"
+    return fx
