@@ -40,3 +40,15 @@ def test_oracle_reproduces_dct_fixture():
     d = dct_blocks()
     assert np.array_equal(bits(oracle.dct8x8(d["blocks"])), bits(d["dct"]))
     assert np.array_equal(bits(oracle.idct8x8(d["blocks"])), bits(d["idct"]))
+
+
+def test_max_grid_fixture_is_current():
+    """The 16384 x 16384 fixture was written for the parameters the GPU test uses, and its
+    coefficient generator is deterministic (checked on a small grid)."""
+    fx = golden_cases.max_grid_case()
+    assert len(fx["sha256"]) == 3 and all(len(h) == 64 for h in fx["sha256"])
+    a = golden_cases.random_420(64, 32, 7, 75)
+    b = golden_cases.random_420(64, 32, 7, 75)
+    assert [c.rounded_px_w for c in a] == [64, 32, 32]
+    for x, y in zip(a, b):
+        assert np.array_equal(x.dct_coefs, y.dct_coefs)
