@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
+import golden_cases
 from golden_cases import dct_blocks, loop_cases
 
 CASES = loop_cases()
