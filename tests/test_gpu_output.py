@@ -91,3 +91,35 @@ def test_process_from_jpeg_bytes(sub, prog, sep):
     ref = oracle.process(j.coefs, 70, 45, [0.3] * 3, [0.001] * 3, [4] * 3, sep, norm_mode="f64")
     assert img.shape == (45, 70, 3)
     assert np.array_equal(img, ref)
+
+
+def test_process_largest_jpeg():
+    """Artefact::process on the largest image the decoder accepts, 16384 x 16384 4:2:0 (3 GiB
+    of planes: the 64-bit-addressed gradient sweep).  The oracle is too slow here, so the
+    check is the shape and closeness to libjpeg's decode after 2 iterations, which move the
+    decoded image by little."""
+    import io
+    from PIL import Image
+    n = 16384
+    yy = np.arange(n, dtype=np.uint32)[:, None]
+    xx = np.arange(n, dtype=np.uint32)[None, :]
+    rgb = np.empty((n, n, 3), np.uint8)
+    rgb[..., 0] = ((xx * 3 + yy) >> 7) & 255
+    rgb[..., 1] = ((xx + yy * 5) >> 8) & 255
+    rgb[..., 2] = 128 + (((xx >> 9) ^ (yy >> 9)) & 63)
+    f = io.BytesIO()
+    Image.fromarray(rgb).save(f, "JPEG", quality=75, subsampling=2)
+    del rgb
+    data = f.getvalue()
+    print(f"case process {n}x{n} 4:2:0, {len(data)} bytes", flush=True)
+    img = (artefact_amd.Artefact().source(artefact_amd.JpegSource.Buffer(data))
+           .iterations(artefact_amd.ForAll(2)).process())
+    assert img.shape == (n, n, 3) and img.dtype == np.uint8
+    limit, Image.MAX_IMAGE_PIXELS = Image.MAX_IMAGE_PIXELS, None  # PIL's guard stops at 179 M pixels
+    try:
+        ref = np.asarray(Image.open(io.BytesIO(data)).convert("RGB"))
+    finally:
+        Image.MAX_IMAGE_PIXELS = limit
+    diff = np.abs(img.astype(np.int16) - ref.astype(np.int16))
+    print(f"mean abs diff {diff.mean():.3f}, max {diff.max()}", flush=True)
+    assert diff.mean() < 3.0
