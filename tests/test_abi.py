@@ -91,3 +91,18 @@ def test_missing_library_fails_loudly(monkeypatch):
         artefact_amd.compute(3, wl["coefs"], 0.3, [0.001] * 3, 2, W, H, W * H)
     with pytest.raises(RuntimeError, match="missing"):
         artefact_amd.Plan(wl["coefs"], 0.3, [0.001] * 3, 2, W, H)
+
+
+def test_every_environment_switch_is_documented():
+    """Each variable the library reads appears in INTEGRATION.md's table of switches."""
+    import glob
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    names = set()
+    for path in glob.glob(os.path.join(root, "artefact_amd", "csrc", "*")):
+        with open(path, errors="replace") as f:
+            names |= set(re.findall(r'getenv\("([A-Z_]+)"\)', f.read()))
+    with open(os.path.join(root, "INTEGRATION.md")) as f:
+        table = f.read().split("## 4. Environment switches")[1].split("\n## ")[0]
+    assert names, "no getenv calls found"
+    missing = sorted(n for n in names if f"`{n}" not in table)
+    assert not missing, missing
