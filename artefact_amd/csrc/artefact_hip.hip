@@ -2303,7 +2303,8 @@ std::map<std::string, TuneChoice> g_tune_cache;
 std::string tune_key(const artefact_hip_plan *pl)
 {
     std::string k = "d" + std::to_string(pl->device) + " " + std::to_string(pl->W) + "x" + std::to_string(pl->H) +
-                    " C" + std::to_string(pl->C) + (pl->tgv ? " tgv" : " tv") + " p" + std::to_string(pl->precision);
+                    " C" + std::to_string(pl->C) + (pl->tgv ? " tgv" : " tv") + " p" + std::to_string(pl->precision) +
+                    (pl->wide ? " w1" : " w0");  // the two k_gradient instances differ in registers and occupancy
     for (uint32_t c = 0; c < pl->C; c++)
         k += " " + std::to_string(pl->ch[c].rw) + "x" + std::to_string(pl->ch[c].rh) + "/" +
              std::to_string(pl->ch[c].hs) + std::to_string(pl->ch[c].vs);

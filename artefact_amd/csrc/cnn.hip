@@ -729,11 +729,20 @@ namespace {
 bool ensure_workspace(artefact_cnn *net, size_t px)
 {
     if (px <= net->cap_px) return true;
-    for (int l = 0; l < NLAYER; l++)
+    // Every pointer is cleared as it is released, and the capacity is 0 until the whole set is
+    // allocated again: after a failed allocation nothing holds a released address, so a later
+    // call (or artefact_cnn_destroy) never frees one twice.
+    net->cap_px = 0;
+    for (int l = 0; l < NLAYER; l++) {
         if (net->act[l]) (void)hipFree(net->act[l]);
-    void *bufs[] = {net->g32a, net->g32b, net->g16, net->resid, net->pred, net->x0, net->target, net->lpart};
-    for (void *p : bufs)
-        if (p) (void)hipFree(p);
+        net->act[l] = nullptr;
+    }
+    void **bufs[] = {(void **)&net->g32a, (void **)&net->g32b, (void **)&net->g16, (void **)&net->resid,
+                     (void **)&net->pred, (void **)&net->x0, (void **)&net->target, (void **)&net->lpart};
+    for (void **p : bufs) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
     bool ok = true;
     ok = ok && hipMalloc(&net->act[0], px * 16 * 2) == hipSuccess;
     for (int l = 1; l < NLAYER; l++) ok = ok && hipMalloc(&net->act[l], px * 32 * 2) == hipSuccess;

@@ -151,22 +151,44 @@ struct BitReader {
 
 int extend(uint32_t v, int s) { return (v < (1u << (s - 1))) ? (int)v - (1 << s) + 1 : (int)v; }
 
-int decode_symbol(BitReader &br, const Huffman &h)
+// Decodes one Huffman symbol.  If `lookup` is given it receives whether the fork's combined
+// run/value table would have decoded an AC symbol (huffman.rs:189-238): a code of at most 9
+// bits whose symbol index is below 255, with magnitude bits s != 0, code length + s <= 9 and an
+// extended value in [-128, 127].  The caller checks the value; here the first three conditions
+// are stored as the symbol's fast-table length (0 when not eligible).
+int decode_symbol(BitReader &br, const Huffman &h, int *fast_len = nullptr)
 {
     const uint32_t look = br.peek(16);
     const uint16_t f = h.fast[look >> 7];
     if (f) {
-        br.drop(f >> 8);
+        const int l = f >> 8;
+        br.drop(l);
+        if (fast_len) {
+            const int code = (int)(look >> (16 - l));
+            const int index = h.valptr[l] + code - h.mincode[l];
+            *fast_len = index < 255 ? l : 0;
+        }
         return f & 0xFF;
     }
     for (int l = 10; l <= 16; l++) {
         const int code = (int)(look >> (16 - l));
         if (code <= h.maxcode[l]) {
             br.drop(l);
+            if (fast_len) *fast_len = 0;
             return h.vals[h.valptr[l] + code - h.mincode[l]];
         }
     }
     throw Error{"bad Huffman code"};
+}
+
+// Natural index at which the fork stores an AC value decoded at zigzag position k, which can
+// pass 63 in a corrupt stream.  Its combined table path clamps (UN_ZIGZAG[min(k, 63)]), its
+// other path wraps (UN_ZIGZAG[k & 63]): bitstream.rs:359-378 (sequential) and :487-503
+// (progressive first scans).
+int ac_position(int k, int fast_len, int s, int value)
+{
+    const bool fast = fast_len != 0 && s != 0 && fast_len + s <= 9 && value >= -128 && value <= 127;
+    return kNatural[fast ? std::min(k, 63) : (k & 63)];
 }
 
 struct Component {
@@ -351,11 +373,13 @@ struct Decoder {
         c.dc_pred = (int32_t)((uint32_t)c.dc_pred + (uint32_t)diff);
         blk[0] = (int16_t)c.dc_pred;
         for (int k = 1; k < 64;) {
-            const int rs = decode_symbol(br, ha);
+            int fl = 0;
+            const int rs = decode_symbol(br, ha, &fl);
             const int r = rs >> 4, s = rs & 15;
             if (s) {
                 k += r;
-                blk[kNatural[std::min(k, 63)]] = (int16_t)extend(br.get(s), s);
+                const int v = extend(br.get(s), s);
+                blk[ac_position(k, fl, s, v)] = (int16_t)v;
                 k++;
             } else if (r != 15) {
                 break;
@@ -447,12 +471,13 @@ struct Decoder {
         const Huffman &ha = ac[c.ta];
         if (!ha.set) throw Error{"missing Huffman table"};
         for (int k = ss; k <= se;) {
-            const int rs = decode_symbol(br, ha);
+            int fl = 0;
+            const int rs = decode_symbol(br, ha, &fl);
             const int r = rs >> 4, s = rs & 15;
             if (s) {
                 k += r;
-                if (k > 63) throw Error{"coefficient index past 63"};
-                blk[kNatural[k]] = (int16_t)((uint16_t)(int16_t)extend(br.get(s), s) << al);
+                const int v = extend(br.get(s), s);
+                blk[ac_position(k, fl, s, v)] = (int16_t)((uint16_t)(int16_t)v << al);
                 k++;
             } else if (r == 15) {
                 k += 16;
@@ -473,7 +498,8 @@ struct Decoder {
         if (eobrun == 0) {
             const Huffman &ha = ac[c.ta];
             if (!ha.set) throw Error{"missing Huffman table"};
-            for (; k <= se; k++) {
+            // k <= se at the top of every pass (bitstream.rs:536-622).
+            while (k <= se) {
                 const int rs = decode_symbol(br, ha);
                 int r = rs >> 4, s = rs & 15;
                 if (s) {
@@ -484,16 +510,21 @@ struct Decoder {
                     if (r) eobrun += (int)br.get(r);
                     break;
                 }
-                do {
+                // Correct the nonzero coefficients passed over, up to the r-th zero one.  A run
+                // that would pass the end of the band stops at its last position, and a new
+                // coefficient then goes there (bitstream.rs:579-616).
+                for (;;) {
                     int16_t &x = blk[kNatural[k]];
                     if (x != 0) {
                         correct(x);
-                    } else {
-                        if (--r < 0) break;
+                    } else if (--r < 0) {
+                        break;
                     }
+                    if (k == se) break;
                     k++;
-                } while (k <= se);
-                if (s && k <= 63) blk[kNatural[k]] = (int16_t)s;
+                }
+                if (s) blk[kNatural[k]] = (int16_t)s;
+                k++;
             }
         }
         if (eobrun > 0) {

@@ -224,12 +224,9 @@ def workload(name: str, seed: int = 0) -> dict:
     }
 
 
-def batch_workload(count: int = 64, seed: int = 0, min_side: int = 256, max_side: int = 2048,
-                   iterations: int = 100) -> list[dict]:
-    """The batched workload: ``count`` images whose width and height are drawn uniformly
-    from the multiples of 16 in [min_side, max_side], half 4:2:0 and half 4:4:4 (shuffled),
-    IJG quality uniform in 30..90, half plain TV and half TGV (weight 0.3), ``iterations``
-    iterations each.  Image i uses generator seed ``seed * 1000 + i``."""
+def batch_specs(count: int = 64, seed: int = 0, min_side: int = 256, max_side: int = 2048,
+                iterations: int = 100) -> list[dict]:
+    """Parameters of the batched workload's images (see ``batch_workload``), without encoding."""
     rng = np.random.default_rng(seed)
     sides = np.arange(min_side, max_side + 1, 16)
     subs = np.array(["420"] * (count // 2) + ["444"] * (count - count // 2))
@@ -240,13 +237,36 @@ def batch_workload(count: int = 64, seed: int = 0, min_side: int = 256, max_side
     for i in range(count):
         w, h = int(rng.choice(sides)), int(rng.choice(sides))
         quality = int(rng.integers(30, 91))
-        coefs = encode(synth_rgb(h, w, seed * 1000 + i), subsampling=str(subs[i]), quality=quality)
-        W = max(c.rounded_px_w for c in coefs)
-        H = max(c.rounded_px_h for c in coefs)
-        mode = str(modes[i])
-        out.append({
-            "name": f"batch{i}_{mode}_{subs[i]}_{w}x{h}_q{quality}", "mode": mode, "coefs": coefs, "width": w,
-            "height": h, "max_w": W, "max_h": H, "quality": quality, "iterations": iterations,
-            "weight": 0.0 if mode == "tv" else 0.3, "pweight": [0.001, 0.001, 0.001], "seed": seed * 1000 + i,
-        })
+        out.append({"index": i, "width": w, "height": h, "subsampling": str(subs[i]), "quality": quality,
+                    "mode": str(modes[i]), "seed": seed * 1000 + i, "iterations": iterations})
     return out
+
+
+def batch_image(spec: dict) -> dict:
+    """One image of the batched workload from its ``batch_specs`` entry."""
+    w, h, mode = spec["width"], spec["height"], spec["mode"]
+    coefs = encode(synth_rgb(h, w, spec["seed"]), subsampling=spec["subsampling"], quality=spec["quality"])
+    W = max(c.rounded_px_w for c in coefs)
+    H = max(c.rounded_px_h for c in coefs)
+    return {
+        "name": f"batch{spec['index']}_{mode}_{spec['subsampling']}_{w}x{h}_q{spec['quality']}", "mode": mode,
+        "coefs": coefs, "width": w, "height": h, "max_w": W, "max_h": H, "quality": spec["quality"],
+        "iterations": spec["iterations"], "weight": 0.0 if mode == "tv" else 0.3, "pweight": [0.001, 0.001, 0.001],
+        "seed": spec["seed"],
+    }
+
+
+def batch_workload(count: int = 64, seed: int = 0, min_side: int = 256, max_side: int = 2048,
+                   iterations: int = 100, workers: int = 1) -> list[dict]:
+    """The batched workload: ``count`` images whose width and height are drawn uniformly
+    from the multiples of 16 in [min_side, max_side], half 4:2:0 and half 4:4:4 (shuffled),
+    IJG quality uniform in 30..90, half plain TV and half TGV (weight 0.3), ``iterations``
+    iterations each.  Image i uses generator seed ``seed * 1000 + i``.  With ``workers`` > 1
+    the images are synthesised by that many spawned processes (host work only)."""
+    specs = batch_specs(count, seed, min_side, max_side, iterations)
+    if workers <= 1:
+        return [batch_image(s) for s in specs]
+    import multiprocessing
+
+    with multiprocessing.get_context("spawn").Pool(workers) as pool:
+        return pool.map(batch_image, specs, chunksize=1)

@@ -132,3 +132,53 @@ def write_baseline(width: int, height: int, components, restart_interval: int = 
     bits.flush()
     out += bits.out + b"\xff\xd9"
     return bytes(out)
+
+
+class ScanBits(_Bits):
+    """Entropy-coded data written symbol by symbol with the writer's fixed-length tables:
+    4-bit DC category codes and 8-bit AC run/size codes."""
+
+    _dc = {s: i for i, s in enumerate(DC_SYMBOLS)}
+    _ac = {s: i for i, s in enumerate(AC_SYMBOLS)}
+
+    def dc(self, diff: int) -> "ScanBits":
+        s = _category(diff)
+        self.put(self._dc[s], 4)
+        if s:
+            self.put(_bits_of(diff, s), s)
+        return self
+
+    def ac(self, run: int, value: int) -> "ScanBits":
+        """Run/size symbol for `value` after `run` zeros; value 0 with run 0 is EOB, run 15 ZRL."""
+        s = _category(value)
+        self.put(self._ac[(run << 4) | s], 8)
+        if s:
+            self.put(_bits_of(value, s), s)
+        return self
+
+    def raw(self, value: int, length: int) -> "ScanBits":
+        self.put(value, length)
+        return self
+
+    def data(self) -> bytes:
+        self.flush()
+        return bytes(self.out)
+
+
+def write_grey_scans(width: int, height: int, quant, scans, progressive: bool = True) -> bytes:
+    """A one-component file with hand-written scans.  scans: list of (ss, se, ah, al, ScanBits)."""
+    q = np.asarray(quant).reshape(-1)
+    out = bytearray(b"\xff\xd8")
+    out += _segment(0xDB, bytes([0]) + bytes(int(q[ZIGZAG[k]]) for k in range(64)))
+    out += _segment(0xC2 if progressive else 0xC0, struct.pack(">BHHB", 8, height, width, 1) + bytes([1, 0x11, 0]))
+    dc_counts = [0] * 16
+    dc_counts[3] = len(DC_SYMBOLS)
+    ac_counts = [0] * 16
+    ac_counts[7] = len(AC_SYMBOLS)
+    out += _segment(0xC4, bytes([0x00]) + bytes(dc_counts) + bytes(DC_SYMBOLS))
+    out += _segment(0xC4, bytes([0x10]) + bytes(ac_counts) + bytes(AC_SYMBOLS))
+    for ss, se, ah, al, bits in scans:
+        out += _segment(0xDA, bytes([1, 1, 0x00, ss, se, (ah << 4) | al]))
+        out += bits.data()
+    out += b"\xff\xd9"
+    return bytes(out)

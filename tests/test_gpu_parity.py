@@ -366,3 +366,19 @@ def test_concurrent_compute_calls_match_sequential():
     for (k, W, H), got in zip(jobs * 2, results):
         ref = oracle.compute([k], 0.3, [0.001] * 3, 12, W, H, norm_mode="f64")
         assert bits_equal(got[0], ref[0])
+
+
+@pytest.mark.parametrize("name,iters", [("tgv_420_96x64", 6), ("tv_luma_64x64", 5), ("tgv_444_72x40", 4)])
+def test_reference_default_init_through_override(name, iters):
+    """The reference's default build runs pipeline_scalar, whose start image unboxes inside the
+    block loop (pipeline_scalar/coef.rs:29-50).  A caller that wants that start image passes it
+    as init_override; the loop from it equals the oracle's loop from the same quirk init."""
+    wl = synth.workload(name, seed=17)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case scalar-quirk init {name} iters={iters}", flush=True)
+    init = [oracle.init_plane(k, W, H, init_mode="scalar_quirk") for k in wl["coefs"]]
+    gpu = run_gpu(wl, iters, init_override=init)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, init_mode="scalar_quirk",
+                         norm_mode="f64")
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"

@@ -19,7 +19,7 @@ from PIL import Image
 
 import artefact_amd
 from artefact_amd import synth
-from jpeg_writer import write_baseline
+from jpeg_writer import ZIGZAG, ScanBits, write_baseline, write_grey_scans
 
 D = synth.dct_matrix()
 
@@ -241,3 +241,52 @@ def test_source_errors_use_the_reference_messages(tmp_path):
         artefact_amd.Artefact().source(artefact_amd.JpegSource.File(str(tmp_path / "missing.jpg"))).process()
     with pytest.raises(artefact_amd.ArtefactProcessError, match="Failed to read JPEG: Failed to decode JPEG"):
         artefact_amd.Artefact().source(artefact_amd.JpegSource.Buffer(b"garbage")).process()
+
+
+def _grey_block(data):
+    j = artefact_amd.decode_jpeg(data)
+    assert j.nchannel == 1
+    return j.coefs[0].dct_coefs.reshape(-1, 64)[0]
+
+
+def test_refinement_run_past_the_band_places_the_new_coefficient_at_its_end():
+    """A refinement symbol whose zero run reaches past the band (zune-jpeg/src/bitstream.rs:579-616):
+    the fork stops the run at the band's last position and puts the new coefficient there."""
+    q = np.full(64, 4)
+    scans = [
+        (0, 0, 0, 0, ScanBits().dc(0)),
+        (1, 5, 0, 1, ScanBits().ac(0, 1).ac(0, 0)),  # zigzag 1 = 1 << 1, then EOB
+        # run 4 with a new +1: zigzag 2..5 hold only four zeros, so the run ends at zigzag 5;
+        # the correction bit of zigzag 1 (passed on the way) is 0.
+        (1, 5, 1, 0, ScanBits().ac(4, 1).raw(0, 1)),
+    ]
+    blk = _grey_block(write_grey_scans(8, 8, q, scans))
+    want = np.zeros(64, np.int16)
+    want[ZIGZAG[1]] = 2
+    want[ZIGZAG[5]] = 1
+    assert np.array_equal(blk, want)
+
+
+@pytest.mark.parametrize("progressive", [False, True])
+@pytest.mark.parametrize("size,where", [(2, "wrap"), (1, "clamp")])
+def test_ac_run_past_63_lands_where_the_fork_puts_it(progressive, size, where):
+    """A run that carries the zigzag position past 63 (corrupt data).  The fork's combined
+    run/value table path clamps the position to 63; its other path wraps it modulo 64
+    (bitstream.rs:359-378 sequential, :487-503 progressive).  With this writer's 8-bit AC codes,
+    a 1-bit value takes the table path (8 + 1 <= 9 bits) and a 2-bit value the other one."""
+    q = np.full(64, 4)
+    ac = ScanBits()
+    if not progressive:
+        ac.dc(0)
+    ac.ac(15, 0).ac(15, 0).ac(15, 0)   # zigzag 1 -> 49
+    ac.ac(12, 1)                       # value 1 at zigzag 61
+    ac.ac(15, size)                    # run 15 from zigzag 62 -> 77
+    if progressive:
+        scans = [(0, 0, 0, 0, ScanBits().dc(0)), (1, 63, 0, 0, ac)]
+    else:
+        scans = [(0, 63, 0, 0, ac)]
+    blk = _grey_block(write_grey_scans(8, 8, q, scans, progressive=progressive))
+    want = np.zeros(64, np.int16)
+    want[ZIGZAG[61]] = 1
+    want[ZIGZAG[77 & 63] if where == "wrap" else ZIGZAG[63]] = size  # value 2 (2 bits) or 1 (1 bit)
+    assert np.array_equal(blk, want)
