@@ -25,9 +25,12 @@ ERRORS = {
     -4: "HIP runtime failure",
     -5: "plan used out of order",
     -6: "malformed or unsupported JPEG data",
+    -7: "grid too large for the no-allocation entry point's static scratch",
     1: "benchmark run: the loop ran and no image was written",
 }
 ARTEFACT_BENCHMARK = 1
+ARTEFACT_E_SCRATCH = -7
+ARTEFACT_SCRATCH_BYTES = 64 << 20
 
 
 class ArtefactError(RuntimeError):
@@ -97,6 +100,8 @@ U8_P = ctypes.POINTER(ctypes.c_uint8)
 # (name, restype, argtypes) of every exported symbol of include/artefact_hip.h.
 SIGNATURES = [
     ("artefact_hip_compute", ctypes.c_int, [ctypes.POINTER(artefact_params), ctypes.POINTER(artefact_channel), FLOAT_PP]),
+    ("artefact_hip_compute_noalloc", ctypes.c_int,
+     [ctypes.POINTER(artefact_params), ctypes.POINTER(artefact_channel), FLOAT_PP, ctypes.c_void_p]),
     ("artefact_hip_plan_create", ctypes.c_int,
      [ctypes.POINTER(artefact_params), ctypes.POINTER(artefact_channel), ctypes.POINTER(ctypes.c_void_p)]),
     ("artefact_hip_plan_init", ctypes.c_int, [ctypes.c_void_p]),

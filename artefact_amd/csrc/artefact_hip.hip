@@ -2265,13 +2265,107 @@ int rows_for_workgroups(uint32_t W, uint32_t H, uint64_t wgs)
 
 int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv, bool fast, bool wide)
 {
+    // An override below 2 rows is ignored: the partials are sized for strips of at least 2.
     if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
         const int r = atoi(e);
-        if (r >= 1 && r <= 4096) return r;
+        if (r >= 2 && r <= 4096) return r;
     }
     int per_cu = 0, cus = 0;
     gradient_residency(device, C, tgv, fast, wide, per_cu, cus);
     return rows_for_workgroups(W, H, (uint64_t)per_cu * cus);
+}
+
+// One step of the FISTA scalar recurrence (pipeline_scalar/mod.rs:50-52): t' = midpoint(1,
+// sqrt(4 t^2 + 1)) (f32::midpoint widened to f64), factor = (t - 1) / t'.  Returns the factor
+// and advances term.
+float fista_factor(float &term)
+{
+    const float nt = (float)(((double)1.0f + (double)sqrtf(4.0f * (term * term) + 1.0f)) / 2.0);
+    const float f = (term - 1.0f) / nt;
+    term = nt;
+    return f;
+}
+
+// The plan's scalar loop parameters, as the reference derives them in f32 (compute_step_tv.rs:70,
+// compute_step.rs:30,47 with compute_step_tv2.rs:88-89, mod.rs:45-52,71).
+void set_loop_scalars(artefact_hip_plan *pl, const artefact_params *p)
+{
+    pl->C = p->nchannel;
+    pl->W = p->max_w;
+    pl->H = p->max_h;
+    pl->N = (size_t)pl->W * pl->H;
+    pl->iterations = p->iterations;
+    pl->weight = p->mode == ARTEFACT_MODE_TV ? 0.0f : p->weight;
+    for (int c = 0; c < 3; c++) pl->pweight[c] = p->pweight[c];
+    // With weight 0 every second-order contribution is +-0 and adding it is a no-op
+    // on the (never -0) accumulator, so the TV-only kernel is bit-identical.
+    pl->tgv = pl->weight != 0.0f;
+    const float Cf = (float)pl->C;
+    pl->alpha_tv = 1.0f / sqrtf(Cf);
+    pl->alpha_tgv = (pl->weight / sqrtf(2.0f)) / sqrtf(Cf);
+    const float radius = sqrtf((float)pl->N) / 2.0f;
+    pl->step = radius / sqrtf(1.0f + (float)pl->iterations);
+}
+
+// Geometry fields of channel c (pointers excluded) and its quantisation tables: q as f32 and the
+// correctly rounded f64 reciprocal of q*q, the product rounded to f32 first (compute_step_prob.rs:29).
+void set_channel(artefact_hip_plan *pl, uint32_t c, const artefact_channel &k, float *quant, double *rqq)
+{
+    double qsum = 0.0;
+    for (int j = 0; j < 64; j++) {
+        const float q = (float)k.quant[j];
+        quant[j] = q;
+        rqq[j] = 1.0 / (double)(q * q);
+        qsum += q;
+    }
+    pl->qmean[c] = (float)(qsum / 64.0);
+    ChanDev &d = pl->ch[c];
+    d.rw = k.rounded_w;
+    d.rh = k.rounded_h;
+    d.bw = k.block_w;
+    d.bh = k.block_h;
+    d.hs = k.h_samp;
+    d.vs = k.v_samp;
+    d.resample = (k.rounded_w != pl->W || k.rounded_h != pl->H) ? 1u : 0u;
+    if (!d.resample) d.mode = PM_FULL;
+    else if (d.hs == 1 && d.vs == 1) d.mode = PM_SUB11;
+    else if (d.hs == 2 && d.vs == 2) d.mode = PM_SUB22;
+    else if (d.hs == 2) d.mode = PM_SUB21;
+    else d.mode = PM_SUB12;
+    d.alpha_p = pl->pweight[c] != 0.0f ? pl->pweight[c] * 2.0f * 255.0f * sqrtf(2.0f) : 0.0f;
+}
+
+// The projection kernel the channels allow (fused layouts, lane-kernel units) and the untuned
+// defaults with their environment overrides.
+void set_layout(artefact_hip_plan *pl)
+{
+    for (uint32_t c = pl->C; c < 3; c++) pl->ch[c] = pl->ch[0];
+    if (pl->C == 1 && pl->ch[0].mode == PM_FULL) pl->layout = LAYOUT_Y;
+    else if (pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_FULL && pl->ch[2].mode == PM_FULL)
+        pl->layout = LAYOUT_444;
+    else if (pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_SUB22 && pl->ch[2].mode == PM_SUB22)
+        pl->layout = LAYOUT_420;
+    else pl->layout = LAYOUT_GENERIC;
+    pl->layout_fused = pl->layout;
+    uint32_t acc = 0;
+    for (uint32_t c = 0; c < 3; c++) {
+        if (c < pl->C) {
+            const uint32_t bpw = pl->ch[c].mode == PM_SUB22 ? 4 : 8;  // blocks per wave, k_project_lane
+            acc += ((pl->ch[c].bw + bpw - 1) / bpw) * pl->ch[c].bh;
+        }
+        pl->lane_units[c] = acc;
+    }
+    // Untuned default: the LDS-free lane kernel, measured on MI355X faster than the LDS kernel
+    // (k_project_fused) on every fused layout (4096x3072 4:4:4: 134 vs 194 us).
+    pl->lane_project = 1;
+    if (const char *e = getenv("ARTEFACT_PROJECT_KERNEL")) pl->lane_project = strcmp(e, "lds") != 0;
+    if (const char *e = getenv("ARTEFACT_FUSE_NORM")) pl->fuse_norm = atoi(e) != 0;
+    if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
+        if (atoi(e)) pl->layout = LAYOUT_GENERIC;
+    // Lets callers of the one-shot entry points (artefact_hip_compute / _process, batches) opt
+    // into the fast gradient arithmetic without an API change.
+    if (const char *e = getenv("ARTEFACT_PRECISION"))
+        if (strcmp(e, "f32_fast") == 0) pl->precision = ARTEFACT_PRECISION_F32_FAST;
 }
 
 // Output stage on `stream` from device planes (y, cb, cr; cb/cr ignored unless colour) into
@@ -2482,6 +2576,7 @@ const char *artefact_hip_strerror(int code)
     case ARTEFACT_E_HIP: return "HIP runtime failure";
     case ARTEFACT_E_STATE: return "plan used out of order";
     case ARTEFACT_E_JPEG: return "malformed or unsupported JPEG data";
+    case ARTEFACT_E_SCRATCH: return "grid too large for the no-allocation entry point's static scratch";
     case ARTEFACT_BENCHMARK: return "benchmark run: no image written";
     default: return "unknown error";
     }
@@ -2499,34 +2594,13 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         delete pl;
         return ARTEFACT_E_HIP;
     }
-    pl->C = p->nchannel;
-    pl->W = p->max_w;
-    pl->H = p->max_h;
-    pl->N = (size_t)pl->W * pl->H;
-    pl->iterations = p->iterations;
-    pl->weight = p->mode == ARTEFACT_MODE_TV ? 0.0f : p->weight;
-    for (int c = 0; c < 3; c++) pl->pweight[c] = p->pweight[c];
-    // With weight 0 every second-order contribution is +-0 and adding it is a no-op
-    // on the (never -0) accumulator, so the TV-only kernel is bit-identical.
-    pl->tgv = pl->weight != 0.0f;
-
-    // f32 constants exactly as the reference derives them (compute_step_tv.rs:70,
-    // compute_step.rs:30,47 with compute_step_tv2.rs:88-89, mod.rs:45-52,71).
-    const float Cf = (float)pl->C;
-    pl->alpha_tv = 1.0f / sqrtf(Cf);
-    pl->alpha_tgv = (pl->weight / sqrtf(2.0f)) / sqrtf(Cf);
-    const float radius = sqrtf((float)pl->N) / 2.0f;
-    pl->step = radius / sqrtf(1.0f + (float)pl->iterations);
+    set_loop_scalars(pl, p);
     // Host vectors sized by the caller's iteration count and grid: an allocation failure is
     // an error code, never an exception through the C interface.
     try {
         pl->factors.resize(pl->iterations);
         float term = 1.0f;
-        for (uint32_t i = 0; i < pl->iterations; i++) {
-            const float nt = (float)(((double)1.0f + (double)sqrtf(4.0f * (term * term) + 1.0f)) / 2.0);
-            pl->factors[i] = (term - 1.0f) / nt;
-            term = nt;
-        }
+        for (uint32_t i = 0; i < pl->iterations; i++) pl->factors[i] = fista_factor(term);
 
         if (p->init_override) {
             pl->init_override.resize(pl->C);
@@ -2597,14 +2671,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     size_t moff = 0;
     for (uint32_t c = 0; c < pl->C; c++) {
         const artefact_channel &k = chs[c];
-        double qsum = 0.0;
-        for (int j = 0; j < 64; j++) {
-            const float q = (float)k.quant[j];
-            quant[c * 64 + j] = q;
-            rqq[c * 64 + j] = 1.0 / (double)(q * q);  // q*q rounded to f32 first, as compute_step_prob.rs:29
-            qsum += q;
-        }
-        pl->qmean[c] = (float)(qsum / 64.0);
+        set_channel(pl, c, k, &quant[c * 64], &rqq[c * 64]);
         ok = ok && hipMemcpyAsync(pl->coef + moff, k.coefs, pl->M[c] * sizeof(int16_t), hipMemcpyHostToDevice,
                                   pl->stream) == hipSuccess;
         ChanDev &d = pl->ch[c];
@@ -2613,50 +2680,9 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         d.prob = pl->prob + moff;
         d.quant = pl->quant + c * 64;
         d.rqq = pl->rqq + c * 64;
-        d.rw = k.rounded_w;
-        d.rh = k.rounded_h;
-        d.bw = k.block_w;
-        d.bh = k.block_h;
-        d.hs = k.h_samp;
-        d.vs = k.v_samp;
-        d.resample = (k.rounded_w != pl->W || k.rounded_h != pl->H) ? 1u : 0u;
-        if (!d.resample) d.mode = PM_FULL;
-        else if (d.hs == 1 && d.vs == 1) d.mode = PM_SUB11;
-        else if (d.hs == 2 && d.vs == 2) d.mode = PM_SUB22;
-        else if (d.hs == 2) d.mode = PM_SUB21;
-        else d.mode = PM_SUB12;
-        d.alpha_p = pl->pweight[c] != 0.0f ? pl->pweight[c] * 2.0f * 255.0f * sqrtf(2.0f) : 0.0f;
         moff += pl->M[c];
     }
-    for (uint32_t c = pl->C; c < 3; c++) pl->ch[c] = pl->ch[0];
-    if (pl->C == 1 && pl->ch[0].mode == PM_FULL) pl->layout = LAYOUT_Y;
-    else if (pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_FULL && pl->ch[2].mode == PM_FULL)
-        pl->layout = LAYOUT_444;
-    else if (pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_SUB22 && pl->ch[2].mode == PM_SUB22)
-        pl->layout = LAYOUT_420;
-    else pl->layout = LAYOUT_GENERIC;
-    pl->layout_fused = pl->layout;
-    {
-        uint32_t acc = 0;
-        for (uint32_t c = 0; c < 3; c++) {
-            if (c < pl->C) {
-                const uint32_t bpw = pl->ch[c].mode == PM_SUB22 ? 4 : 8;  // blocks per wave, k_project_lane
-                acc += ((pl->ch[c].bw + bpw - 1) / bpw) * pl->ch[c].bh;
-            }
-            pl->lane_units[c] = acc;
-        }
-    }
-    // Untuned default: the LDS-free lane kernel, measured on MI355X faster than the LDS kernel
-    // (k_project_fused) on every fused layout (4096x3072 4:4:4: 134 vs 194 us).
-    pl->lane_project = 1;
-    if (const char *e = getenv("ARTEFACT_PROJECT_KERNEL")) pl->lane_project = strcmp(e, "lds") != 0;
-    if (const char *e = getenv("ARTEFACT_FUSE_NORM")) pl->fuse_norm = atoi(e) != 0;
-    if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
-        if (atoi(e)) pl->layout = LAYOUT_GENERIC;
-    // Lets callers of the one-shot entry points (artefact_hip_compute / _process, batches) opt
-    // into the fast gradient arithmetic without an API change.
-    if (const char *e = getenv("ARTEFACT_PRECISION"))
-        if (strcmp(e, "f32_fast") == 0) pl->precision = ARTEFACT_PRECISION_F32_FAST;
+    set_layout(pl);
     ok = ok && hipMemcpyAsync(pl->quant, quant.data(), quant.size() * sizeof(float), hipMemcpyHostToDevice,
                               pl->stream) == hipSuccess;
     ok = ok && hipMemcpyAsync(pl->rqq, rqq.data(), rqq.size() * sizeof(double), hipMemcpyHostToDevice,
@@ -3241,6 +3267,151 @@ int artefact_hip_compute(const artefact_params *p, const artefact_channel *ch, f
     if (!rc) rc = artefact_hip_plan_download(pl, out);
     artefact_hip_plan_destroy(pl);
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// The loop without any allocation (artefact_hip_compute_noalloc).  Every buffer beyond the
+// caller's coefficients and output planes -- x and x_prev, the extrapolated image and the
+// gradient, the coefficient estimate and the prob plane, the quantisation tables, the norm
+// partials and norms -- is carved from one static device array, which each device that runs
+// the library holds from the first call until the process ends.  Host code never passes the
+// array's name to a kernel (that is a host-side stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress once per device, under g_scratch_mu.
+static __device__ __align__(256) unsigned char scratch[ARTEFACT_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[SCRATCH_DEVICES] = {};
+
+// Device address of this device's copy of `scratch` (the current device), or nullptr.
+void *scratch_address(int device)
+{
+    if (device < 0 || device >= SCRATCH_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+// The quantisation tables as kernel arguments, written into the scratch tables by k_tables.
+struct TableArgs {
+    float quant[3][64];
+    double rqq[3][64];
+};
+__global__ __launch_bounds__(64) void k_tables(const TableArgs t, float *quant, double *rqq)
+{
+    const int j = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        quant[c * 64 + j] = t.quant[c][j];
+        rqq[c * 64 + j] = t.rqq[c][j];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int artefact_hip_compute_noalloc(const artefact_params *p, const artefact_channel *ch, float *const *out,
+                                 void *stream)
+{
+    int rc = check_inputs(p, ch);
+    if (rc) return rc;
+    if (!out || p->init_override) return ARTEFACT_E_ARGUMENT;
+    for (uint32_t c = 0; c < p->nchannel; c++) {
+        if (!out[c]) return ARTEFACT_E_ARGUMENT;
+        // The projection kernels read a block row of coefficients as one 16-byte load.
+        if (reinterpret_cast<uintptr_t>(ch[c].coefs) % 16) return ARTEFACT_E_ARGUMENT;
+    }
+    // The plan lives on the stack: its host vectors stay empty, so nothing is allocated.
+    artefact_hip_plan pl;
+    pl.stream = static_cast<hipStream_t>(stream);
+    set_loop_scalars(&pl, p);
+    size_t mtot = 0;
+    for (uint32_t c = 0; c < pl.C; c++) {
+        pl.M[c] = (size_t)ch[c].rounded_w * ch[c].rounded_h;
+        mtot += pl.M[c];
+    }
+    // Partials for the smallest strip height (2 rows), so that the size check needs no device query.
+    const size_t nwg = (size_t)((pl.W + KG_COLS - 1) / KG_COLS) * ((pl.H + WAVES * 2 - 1) / (WAVES * 2));
+
+    // Scratch layout, each buffer at a 256-byte boundary, as the plan arena.
+    const size_t cn = (size_t)pl.C * pl.N;
+    size_t bytes = 0;
+    auto take = [&bytes](size_t n) {
+        const size_t at = bytes;
+        bytes += (n + 255) & ~(size_t)255;
+        return at;
+    };
+    const size_t at_x = take(cn * 4), at_xp = take(cn * 4), at_g = take(cn * 4), at_y = take(cn * 4);
+    const size_t at_cos = take(mtot * 4), at_prob = take(mtot * 4);
+    const size_t at_quant = take(3 * 64 * 4), at_rqq = take(3 * 64 * 8), at_part = take(nwg * 6 * 8);
+    const size_t at_norms = take(3 * 4), at_rnorm = take(3 * 8), at_nsq = take(3 * 8);
+    if (bytes > ARTEFACT_SCRATCH_BYTES) return ARTEFACT_E_SCRATCH;
+    if (hipGetDevice(&pl.device) != hipSuccess) return ARTEFACT_E_HIP;
+    const char *wide_env = getenv("ARTEFACT_WIDE_ADDRESSING");
+    pl.wide = wide_env && wide_env[0] == '1';
+    pl.rows = pick_rows(pl.device, pl.W, pl.H, pl.C, pl.tgv, false, pl.wide);
+    pl.grid_g = dim3((pl.W + KG_COLS - 1) / KG_COLS, (pl.H + WAVES * pl.rows - 1) / (WAVES * pl.rows));
+    pl.grid_p = dim3((pl.W + TW - 1) / TW, (pl.H + TH - 1) / TH);
+    char *base = static_cast<char *>(scratch_address(pl.device));
+    if (!base) return ARTEFACT_E_HIP;
+    pl.buf[0] = reinterpret_cast<float *>(base + at_x);
+    pl.buf[1] = reinterpret_cast<float *>(base + at_xp);
+    pl.g = reinterpret_cast<float *>(base + at_g);
+    pl.ybuf = reinterpret_cast<float *>(base + at_y);
+    pl.cos = reinterpret_cast<float *>(base + at_cos);
+    pl.prob = reinterpret_cast<float *>(base + at_prob);
+    pl.quant = reinterpret_cast<float *>(base + at_quant);
+    pl.rqq = reinterpret_cast<double *>(base + at_rqq);
+    pl.partials = reinterpret_cast<double *>(base + at_part);
+    pl.norms = reinterpret_cast<float *>(base + at_norms);
+    pl.rnorm = reinterpret_cast<double *>(base + at_rnorm);
+    pl.norm_sq = reinterpret_cast<double *>(base + at_nsq);
+
+    TableArgs tables;
+    for (int c = 0; c < 3; c++)
+        for (int j = 0; j < 64; j++) {
+            tables.quant[c][j] = 1.0f;
+            tables.rqq[c][j] = 1.0;
+        }
+    size_t moff = 0;
+    for (uint32_t c = 0; c < pl.C; c++) {
+        set_channel(&pl, c, ch[c], tables.quant[c], tables.rqq[c]);
+        ChanDev &d = pl.ch[c];
+        d.coef = ch[c].coefs;
+        d.cos = pl.cos + moff;
+        d.prob = pl.prob + moff;
+        d.quant = pl.quant + c * 64;
+        d.rqq = pl.rqq + c * 64;
+        moff += pl.M[c];
+    }
+    set_layout(&pl);
+
+    // Every buffer the loop reads is written first (tables, start image, cos, prob = 0), so the
+    // previous call's contents need no clearing.
+    hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, pl.stream, tables, pl.quant, pl.rqq);
+    launch_init(&pl, pl.stream);
+    float term = 1.0f;
+    int cur = 0;
+    for (uint32_t it = 0; it < pl.iterations; it++) {
+        const IterArgs a = make_args(&pl, cur, fista_factor(term), it + 1 == pl.iterations);
+        launch_gradient(&pl, a, pl.stream);
+        launch_project(&pl, a, pl.stream);
+        cur = 1 - cur;
+    }
+    bool ok = hipGetLastError() == hipSuccess;
+    for (uint32_t c = 0; c < pl.C && ok; c++)
+        ok = hipMemcpyAsync(out[c], pl.buf[cur] + c * pl.N, pl.N * sizeof(float), hipMemcpyDeviceToDevice,
+                            pl.stream) == hipSuccess;
+    return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
 
 }  // extern "C"

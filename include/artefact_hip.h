@@ -34,6 +34,7 @@ extern "C" {
 #define ARTEFACT_E_HIP          -4  /* HIP runtime failure (allocation, launch, copy) */
 #define ARTEFACT_E_STATE        -5  /* plan used out of order (iterate before init) */
 #define ARTEFACT_E_JPEG         -6  /* malformed or unsupported JPEG data (artefact_jpeg_decode) */
+#define ARTEFACT_E_SCRATCH      -7  /* artefact_hip_compute_noalloc: the grid's buffers exceed ARTEFACT_SCRATCH_BYTES */
 #define ARTEFACT_BENCHMARK       1  /* artefact_hip_process with benchmark set: the loop ran, no image was
                                        written (the reference returns Err("BENCHMARK"), lib.rs:155-157) */
 
@@ -263,6 +264,29 @@ int artefact_jpeg_size(const artefact_jpeg *jpeg, uint32_t *real_w, uint32_t *re
  * they can go straight to artefact_hip_process or artefact_hip_compute. */
 int artefact_jpeg_channels(const artefact_jpeg *jpeg, artefact_channel *out_channels);
 void artefact_jpeg_free(artefact_jpeg *jpeg);
+
+/* ---- No-allocation form of artefact_hip_compute ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, hosts that forbid allocation after start-up.  The
+ * same loop as artefact_hip_compute (the same kernels, bit-identical results), but
+ *   - channels[c].coefs are DEVICE pointers (16-byte aligned int16 coefficients, layout as
+ *     artefact_channel), channels[c].quant stays a host pointer (read during the call);
+ *   - out_planes[c] are DEVICE pointers to nchannel planes of max_w*max_h f32;
+ *   - every other buffer is carved from one static device array of ARTEFACT_SCRATCH_BYTES that
+ *     each device running the library holds until the process ends; no hipMalloc, hipFree or
+ *     other allocating routine is called (the first call on a device looks up the array's
+ *     address);
+ *   - the kernels and a device-to-device copy into out_planes are queued on `stream` (a
+ *     hipStream_t; NULL is the null stream) and the call returns without waiting;
+ *   - init_override must be NULL; the strip height and projection kernel are the untuned
+ *     defaults (the results do not depend on them).
+ * Grids whose buffers (16 * nchannel * max_w * max_h bytes plus 8 bytes per coefficient and
+ * small tables) exceed ARTEFACT_SCRATCH_BYTES return ARTEFACT_E_SCRATCH: 1024x1024 4:2:0 colour
+ * or 1664x1664 luma fit.  Calls on one device share its array and keep the last call's
+ * contents: two calls whose work overlaps on one device (other streams or threads) race. */
+#define ARTEFACT_SCRATCH_BYTES (64u << 20)
+int artefact_hip_compute_noalloc(const artefact_params *params, const artefact_channel *channels,
+                                 float *const *out_planes, void *stream);
 
 const char *artefact_hip_strerror(int code);
 

@@ -106,3 +106,23 @@ def test_every_environment_switch_is_documented():
     assert names, "no getenv calls found"
     missing = sorted(n for n in names if f"`{n}" not in table)
     assert not missing, missing
+
+
+def test_noalloc_rejects_grids_past_the_scratch():
+    """artefact_hip_compute_noalloc checks the grid against its static scratch array before any
+    device work: 4:2:0 2048x1536 needs about 188 MB, past the 64 MiB array."""
+    from artefact_amd import synth
+    from artefact_amd.compute import _channels, _params
+
+    lib = _lib.load()
+    wl = synth.workload("tgv_420_2048", seed=0)
+    chans, keep = _channels(wl["coefs"])
+    fake = ctypes.c_void_p(256)  # 16-byte aligned and never dereferenced: the size check comes first
+    for c in range(3):
+        chans[c].coefs = ctypes.cast(fake, ctypes.POINTER(ctypes.c_int16))
+    outs = (ctypes.POINTER(ctypes.c_float) * 3)(*[ctypes.cast(fake, ctypes.POINTER(ctypes.c_float))
+                                                  for _ in range(3)])
+    p = _params(3, 0.3, [0.001] * 3, 100, wl["max_w"], wl["max_h"], "tgv", None, keep)
+    rc = lib.artefact_hip_compute_noalloc(ctypes.byref(p), chans, ctypes.cast(outs, _lib.FLOAT_PP), None)
+    assert rc == _lib.ARTEFACT_E_SCRATCH
+    assert lib.artefact_hip_strerror(rc).startswith(b"grid too large")
