@@ -12,9 +12,11 @@
  *              max_rounded_px_count: usize) -> Vec<Vec<f32>>
  *
  * Plain pointers and sizes only.  Every call is re-entrant: a plan owns its own
- * HIP stream and device buffers, and the library holds no global mutable state,
- * so concurrent calls from several host threads (the reference's separate-
- * components mode fans out over rayon workers) are safe.
+ * HIP stream and device buffers, and the library's only shared state is the
+ * autotuner's per-geometry cache (behind a mutex), so concurrent calls from several
+ * host threads (the reference's separate-components mode fans out over rayon
+ * workers) are safe.  The exception is artefact_hip_compute_noalloc, whose calls
+ * share one static scratch array per device (see there).
  */
 #ifndef ARTEFACT_HIP_H
 #define ARTEFACT_HIP_H
