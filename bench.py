@@ -12,9 +12,10 @@ The JSON line also carries
   coefficient estimate read and written per component coefficient) divided by
   the measured mean duration of one iteration's two kernels (HIP events around
   each launch on the plan's stream), against 8 TB/s;
-* ``cpu_baseline``: the CPU oracle (a single-threaded restatement of the
-  reference's scalar loop) timed on this host on a bounded sample of the same
-  workload (same geometry, 30 iterations, loop only).
+* ``cpu_baseline``: the CPU oracle (a restatement of the reference's scalar loop, with
+  its threading: the FISTA update one thread per channel, the rest serial) timed on this
+  host on a bounded sample of the same workload (same geometry, 30 iterations, loop only),
+  with the single-threaded figure beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -168,35 +169,55 @@ def fast_comparison(wl, args, exact_out, exact_objective, B):
 
 
 def host_cpu():
-    """(CPU model name, CPUs this process may run on)."""
-    model = None
+    """(CPU description, CPUs this process may run on).  The description is the first
+    processor's model name with its family, model, stepping and clock from /proc/cpuinfo (a
+    virtual machine may report only a generic model name)."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                if not line.strip():
+                    break  # first processor only
+                k, _, v = line.partition(":")
+                fields[k.strip()] = v.strip()
     except OSError:
         pass
+    model = fields.get("model name")
+    extra = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping", "cpu MHz") if k in fields]
+    if model and extra:
+        model = f"{model} ({', '.join(extra)})"
     return model, len(os.sched_getaffinity(0))
 
 
 def cpu_baseline(wl, iters, runs=3):
+    """The oracle's loop on this host with the reference's own threading: one thread per channel
+    for the FISTA update (pipeline_scalar/mod.rs:54-59, rayon), compute_step serial
+    (compute_step.rs:19-71).  The single-threaded figure is reported beside it."""
     import oracle
 
     W, H = wl["max_w"], wl["max_h"]
-    secs = min(oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, return_time=True)[1]
-               for _ in range(runs))
+    C = len(wl["coefs"])
+
+    def best(threads):
+        return min(oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, return_time=True,
+                                  threads=threads)[1] for _ in range(runs))
+
+    secs = best(C)
+    secs1 = best(1) if C > 1 else secs
     model, ncpu = host_cpu()
     return {
         "value": W * H * iters / secs / 1e6,
         "unit": "Mpixel*it/s",
-        "cores": 1,
+        "cores": C,
         "kind": "port",
+        "single_thread_value": W * H * iters / secs1 / 1e6,
         "cpu_model": model,
         "host_cpus": ncpu,
-        "sample": f"{describe(wl)} geometry, {iters} iterations, loop only, best of {runs} runs ({secs:.2f} s), "
-                  f"single-threaded C restatement of pipeline_scalar (oracle/oracle.c, -O2 -ffp-contract=off)",
+        "sample": f"{describe(wl)} geometry, {iters} iterations, loop only, best of {runs} runs ({secs:.2f} s; "
+                  f"{secs1:.2f} s on one thread), C restatement of pipeline_scalar (oracle/oracle.c, -O2 "
+                  f"-ffp-contract=off) with the reference's threading: the FISTA update on {C} thread(s), one per "
+                  f"channel (mod.rs:54-59), the prob, TV, second-order, norm, descent and projection steps serial "
+                  f"(compute_step.rs:19-71)",
     }
 
 

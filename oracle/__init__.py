@@ -61,6 +61,7 @@ _SIGS = {
     "oracle_objective": (ctypes.c_double, [_u32, _CH, ctypes.c_float, _F, _u32, _u32, _FPP, _D]),
     "oracle_objective_f64planes": (ctypes.c_double, [_u32, _CH, ctypes.c_float, _F, _u32, _u32, _DPP, _D]),
     "oracle_output_rgb": (None, [_u32, _FPP, _u32, _u32, _u32, ctypes.POINTER(ctypes.c_uint8)]),
+    "oracle_set_threads": (None, [ctypes.c_int]),
 }
 
 _lib = None
@@ -136,9 +137,13 @@ def _check(rc, where):
 
 
 def compute(coefs, weight, pweight, iterations, max_w, max_h, *, nchannel=None, init_mode="corrected",
-            norm_mode="reference", init_override=None, return_norms=False, return_time=False):
-    """f32 loop of pipeline_scalar/mod.rs:17-78.  Returns nchannel planes (N f32 each)."""
+            norm_mode="reference", init_override=None, return_norms=False, return_time=False, threads=1):
+    """f32 loop of pipeline_scalar/mod.rs:17-78.  Returns nchannel planes (N f32 each).
+
+    ``threads`` > 1 runs the loops the reference runs over channels with rayon (the FISTA
+    update, mod.rs:54-59) one thread per channel; compute_step stays serial, as there."""
     lib = load()
+    lib.oracle_set_threads(int(threads))
     C = nchannel or len(coefs)
     chans, keep = _chan_array(coefs[:C])
     N = max_w * max_h
@@ -166,6 +171,7 @@ def compute_f64(coefs, weight, pweight, iterations, max_w, max_h, *, nchannel=No
                 init_override=None, return_time=False):
     """The same loop in f64 (norm in f64), the FP64 reference for the precision form."""
     lib = load()
+    lib.oracle_set_threads(1)
     C = nchannel or len(coefs)
     chans, keep = _chan_array(coefs[:C])
     N = max_w * max_h

@@ -34,6 +34,13 @@
 #include <string.h>
 #include <time.h>
 
+/* Threads for the loops the reference runs over channels with rayon (pipeline_scalar/mod.rs:
+ * 27-30 coefficient conversion, 54-59 FISTA update, 77 collection); everything inside
+ * compute_step stays serial, as there.  1 unless oracle_set_threads says otherwise. */
+static int g_channel_threads = 1;
+
+void oracle_set_threads(int n) { g_channel_threads = n < 1 ? 1 : n; }
+
 static double now_seconds(void)
 {
     struct timespec ts;

@@ -56,6 +56,8 @@ double oracle_objective(uint32_t nchannel, const oracle_channel *ch, float weigh
 double oracle_objective_f64planes(uint32_t nchannel, const oracle_channel *ch, float weight,
                                   const float *pweight, uint32_t max_w, uint32_t max_h,
                                   const double *const *planes, double *terms);
+/* Threads for the loops the reference runs over channels (rayon); 1 by default. */
+void oracle_set_threads(int n);
 
 #ifdef __cplusplus
 }

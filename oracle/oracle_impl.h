@@ -533,6 +533,8 @@ static int FN(compute)(uint32_t C, const oracle_channel *ch, float weight, const
                 factor = (term - (R)1) / next_term;
                 step_size = radius / (R)sqrt(1.0 + (double)iterations);
             }
+            /* One thread per channel when asked for, as the reference's par_iter_mut (mod.rs:54-59). */
+#pragma omp parallel for num_threads(g_channel_threads) if (g_channel_threads > 1) schedule(static, 1)
             for (uint32_t c = 0; c < C; c++) {
                 for (size_t i = 0; i < N; i++)
                     aux[c].fista[i] = factor * (aux[c].fdata[i] - aux[c].fista[i]) + aux[c].fdata[i];

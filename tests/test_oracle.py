@@ -432,3 +432,14 @@ def test_objective_closed_forms():
     ap = float(pw * f32(2) * f32(255) * np.sqrt(f32(2)))
     q0 = float(k.quant_table[0])
     assert abs(terms[2] - 0.5 * ap * 4 * (8 * 3.0 / q0) ** 2) < 1e-6
+
+
+def test_channel_threads_give_the_same_bits():
+    """Three threads for the FISTA update (one per channel, as the reference's rayon loop over
+    channels, pipeline_scalar/mod.rs:54-59) change nothing but the timing."""
+    wl = synth.workload("tgv_420_96x64", seed=4)
+    W, H = wl["max_w"], wl["max_h"]
+    one = oracle.compute(wl["coefs"], 0.3, wl["pweight"], 7, W, H, threads=1)
+    three = oracle.compute(wl["coefs"], 0.3, wl["pweight"], 7, W, H, threads=3)
+    for a, b in zip(one, three):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
