@@ -164,24 +164,15 @@ __device__ __forceinline__ void dct8(float v[8])
 // FISTA extrapolation, pipeline_scalar/mod.rs:56: factor*(x - x_prev) + x.
 __device__ __forceinline__ float fista(float x, float xp, float f) { return f * (x - xp) + x; }
 
-// f32::midpoint as Rust std computes it on x86_64: widened to f64, halved, rounded once.
-__device__ __forceinline__ float midpoint(float a, float b)
-{
-    return (float)(((double)a + (double)b) * 0.5);
-}
-
-// midpoint() with an f32 fast path.  When the f32 sum is finite and its half is a normal
-// number or zero, fl32(a + b) / 2 is exact and equals the widened form: either a + b is
-// exact in f64, or |b| < 2^-29 |a| and both forms round to a.  Otherwise (a subnormal or
-// infinite half, NaN) the whole wave takes the widened form.
-__device__ __forceinline__ float midpoint_fast(float a, float b)
-{
-    float m = (a + b) * 0.5f;
-    // v_cmp_class mask: NaN (bits 0-1), +-inf (2, 9), +-subnormal (4, 7).
-    const bool slow = __builtin_amdgcn_class(m, 0x297);
-    if (__builtin_amdgcn_ballot_w64(slow)) m = slow ? midpoint(a, b) : m;
-    return m;
-}
+// f32::midpoint, which Rust std computes on x86_64 widened to f64, halved and rounded once:
+// (float)(((double)a + (double)b) * 0.5).  In f32 arithmetic, fl32(fl32(a + b) * 0.5) equals
+// that widened form for every pair whose sum stays below 2^128.  If a + b is exact in f32, both
+// round the exact half once (the f64 sum and halving are exact).  If it is not, |a + b| >=
+// 2^-125 (below that both operands are multiples of 2^-149, or Sterbenz applies, and the sum is
+// exact), so the f32 half is exact and both forms equal RN32(a + b) / 2: the f64 sum is exact
+// unless |b| < 2^-29 |a|, and then both round to a.  The iterates here stay far below 2^128 (see
+// the header comment).
+__device__ __forceinline__ float midpoint_f32(float a, float b) { return (a + b) * 0.5f; }
 
 // Rust's f32::clamp(lo, hi): lower bound first, NaN passes through.
 __device__ __forceinline__ float clamp_ref(float v, float lo, float hi)
@@ -459,10 +450,13 @@ __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
                     if (TGV) {
                         const float lgx = from_left(gx), lgy = from_left(gy);
                         const float xx = xl ? gx - lgx : 0.0f;
-                        const float yx = xl ? gy - lgy : 0.0f;
+                        // No select: at x = 0 the lane to the left (column -1) loaded column 0, so
+                        // its y-difference equals this one bit for bit and yx is +0, the value the
+                        // reference's x > 0 test gives (compute_step_tv2.rs:53-58).
+                        const float yx = gy - lgy;
                         const float yy = top ? 0.0f : gy - dys1[c];
                         const float xy = top ? 0.0f : gx - dxs1[c];
-                        const float sm = midpoint_fast(xy, yx);
+                        const float sm = midpoint_f32(xy, yx);
                         n2acc += yy * yy + (2.0f * (sm * sm) + xx * xx);
                         gxx[c] = xx;
                         gyy[c] = yy;

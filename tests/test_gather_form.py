@@ -44,3 +44,37 @@ def test_gather_equals_scatter(name, seed):
     gm = gather_gradient([p.reshape(H, W) for p in planes], atv, atg, prob_planes(wl["coefs"], cos, pw, W, H))
     for a, b in zip(g, gm):
         assert np.array_equal(a.view(np.uint32), b.reshape(-1).view(np.uint32))
+
+
+def _widened_midpoint(a, b):
+    """f32::midpoint as the reference restates it: widened to f64, halved, rounded once."""
+    return ((a.astype(np.float64) + b.astype(np.float64)) * 0.5).astype(f32)
+
+
+def test_f32_midpoint_equals_widened_form():
+    """k_gradient forms the symmetric second difference as fl32(fl32(a + b) * 0.5); that equals
+    the widened f64 form for every pair whose sum stays below 2^128.  Checked on random bit
+    patterns over the whole exponent range (subnormals included), on cancelling pairs, on tiny
+    pairs whose half is subnormal and on pairs with wide exponent gaps."""
+    rng = np.random.default_rng(7)
+    n = 1 << 21
+    bits = rng.integers(0, 1 << 32, size=(2, n), dtype=np.uint64).astype(np.uint32)
+    a, b = bits.view(f32)
+    keep = np.isfinite(a) & np.isfinite(b) & (np.abs(a) < f32(1.5e38)) & (np.abs(b) < f32(1.5e38))
+    cases = [(a[keep], b[keep])]
+    # cancelling pairs: b = -a nudged by a few ulps
+    x = rng.uniform(-1e30, 1e30, n).astype(f32)
+    nudge = rng.integers(-4, 5, n).astype(np.int32)
+    y = (-x).view(np.int32) + nudge
+    cases.append((x, y.view(f32)))
+    # tiny operands: sums and halves in and around the subnormal range
+    t = (rng.integers(-(1 << 26), 1 << 26, size=(2, n)).astype(np.float64) * 2.0 ** -149).astype(f32)
+    cases.append((t[0], t[1]))
+    # wide exponent gaps
+    big = rng.uniform(-1e6, 1e6, n).astype(f32)
+    small = (rng.uniform(-1, 1, n) * 10.0 ** rng.uniform(-45, -2, n)).astype(f32)
+    cases.append((big, small))
+    for p, q in cases:
+        got = (p + q) * f32(0.5)
+        want = _widened_midpoint(p, q)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
