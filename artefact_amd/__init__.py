@@ -9,7 +9,8 @@ from .batch import Batch, BatchImage
 from .compute import Plan, compute
 from .process import Artefact, ArtefactProcessError, ForAll, ForEach, Jpeg, JpegSource, process_coefficients
 from .jpeg import JpegDecodeError, decode as decode_jpeg
+from .image import ImageFormatError, encode as encode_image, save as save_image
 from .types import Coefficient, max_grid
 
 __all__ = ["Batch", "BatchImage", "Artefact", "ArtefactError", "ArtefactProcessError", "Coefficient", "ForAll", "ForEach", "Jpeg",
-           "JpegSource", "JpegDecodeError", "decode_jpeg", "LIB_PATH", "Plan", "compute", "max_grid", "process_coefficients"]
+           "JpegSource", "JpegDecodeError", "decode_jpeg", "ImageFormatError", "encode_image", "save_image", "LIB_PATH", "Plan", "compute", "max_grid", "process_coefficients"]

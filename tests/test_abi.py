@@ -21,7 +21,7 @@ def declared_functions():
         if fn.endswith(".h"):
             text = open(os.path.join(inc, fn)).read()
             text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-            names |= set(re.findall(r"\b(artefact_hip_\w+)\s*\(", text))
+            names |= set(re.findall(r"\b(artefact_\w+)\s*\(", text))
     return names
 
 
