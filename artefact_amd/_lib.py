@@ -30,6 +30,9 @@ ERRORS = {
 }
 ARTEFACT_BENCHMARK = 1
 ARTEFACT_E_SCRATCH = -7
+ARTEFACT_IMAGE_PNG = 0
+ARTEFACT_IMAGE_BMP = 1
+ARTEFACT_IMAGE_TIFF = 2
 ARTEFACT_SCRATCH_BYTES = 64 << 20
 
 
@@ -100,6 +103,10 @@ U8_P = ctypes.POINTER(ctypes.c_uint8)
 # (name, restype, argtypes) of every exported symbol of include/artefact_hip.h.
 SIGNATURES = [
     ("artefact_hip_compute", ctypes.c_int, [ctypes.POINTER(artefact_params), ctypes.POINTER(artefact_channel), FLOAT_PP]),
+    ("artefact_image_encode_bound", ctypes.c_size_t, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+    ("artefact_image_encode", ctypes.c_int,
+     [U8_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, U8_P, ctypes.c_size_t,
+      ctypes.POINTER(ctypes.c_size_t)]),
     ("artefact_hip_compute_noalloc", ctypes.c_int,
      [ctypes.POINTER(artefact_params), ctypes.POINTER(artefact_channel), FLOAT_PP, ctypes.c_void_p]),
     ("artefact_hip_plan_create", ctypes.c_int,

@@ -296,7 +296,7 @@ using IC = std::integral_constant<int, N>;
 // based at that row's 64-bit address, with the lane offset alone: the same clamped loads,
 // zero loads and dropped stores, with offsets below one row's bytes.
 template <int C, bool TGV, bool FAST = false, bool WIDE = false>
-__global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
+__global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
 {
     __shared__ DD lred[WAVES][3][64];
 
@@ -423,6 +423,26 @@ __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
         float *yt = yring[km], *dxt = dxring[km];
 #pragma unroll
         for (int c = 0; c < C; c++) yt[c] = fista(nx[c], nxp[c], f);
+        {
+            // Store y of row t at once if the row is one of this strip's outputs (otherwise the
+            // lane offset moves out of range: a dropped store), so no row of y stays in registers
+            // for the two steps until its gradient is done.
+            const bool tin = t >= r0 && t < r0 + RW && t < H;
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                if constexpr (WIDE) {
+                    const size_t ro = tin ? (size_t)t * (size_t)W : 0;
+                    const __amdgpu_buffer_rsrc_t r2 = plane_rsrc(a.y[c] + ro, rowb);
+                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yt[c]), r2,
+                                                          (int)(tin ? vout : rowb), 0, 0);
+                } else {
+                    const uint32_t toff = tin ? (uint32_t)t * (uint32_t)W * 4u : 0u;
+                    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yt[c]), ry,
+                                                          (int)(tin ? vout : arena),
+                                                          (int)(toff + (uint32_t)c * pbytes), 0);
+                }
+            }
+        }
         if (k + 1 < kend) {
             rowload(t + 1);
             if constexpr (ph >= 1) probload(t + 1, pring[k2]);  // the head of step k + 1 (row t + 1)
@@ -529,9 +549,8 @@ __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
                 // above: the head), takes its own row's terms with the row above's tgy and tgd
                 // (the body), and ends with the row below's tgd (from the left) and tgy.
                 if constexpr (ph >= 4) {
-                    // Finish output row q = s - 1 and store it with its y.
+                    // Finish output row q = s - 1 and store it.
                     const int q = s - 1;
-                    const float *yq = yring[k2];  // y of row q = t - 2
                     const bool qin = q < H;
                     const bool acc_on = out_lane && qin;
                     // Branch-free: a row outside the image moves the lane offset out of range
@@ -550,20 +569,17 @@ __global__ __launch_bounds__(NT, FAST ? 5 : 4) void k_gradient(const IterArgs a)
                             const size_t ro = qin ? (size_t)q * (size_t)W : 0;
                             const uint32_t vw = qin ? vout : rowb;
                             const __amdgpu_buffer_rsrc_t r1 = plane_rsrc(a.g[c] + ro, rowb);
-                            const __amdgpu_buffer_rsrc_t r2 = plane_rsrc(a.y[c] + ro, rowb);
                             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), r1, (int)vw, 0, 0);
-                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yq[c]), r2, (int)vw, 0,
-                                                                  0);
                         } else {
                             const int so = (int)(qoff + (uint32_t)c * pbytes);
                             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, gv), rg, (int)vst, so,
                                                                   0);
-                            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, yq[c]), ry, (int)vst,
-                                                                  so, 0);
                         }
+                        // A lane's sum of g^2 over its strip rows is a plain f64 sum (g^2 is exact in
+                        // f64); lanes, waves and workgroups are merged in double-double (the
+                        // exact mode) or f64 (the fast mode).
                         const float ga = acc_on ? gv : 0.0f;
-                        if constexpr (FAST) acc[c].hi = __builtin_fma((double)ga, (double)ga, acc[c].hi);
-                        else dd_add(acc[c], (double)ga * (double)ga);
+                        acc[c].hi = __builtin_fma((double)ga, (double)ga, acc[c].hi);
                     }
                 }
                 if constexpr (ph >= 3) {

@@ -267,6 +267,23 @@ int artefact_jpeg_size(const artefact_jpeg *jpeg, uint32_t *real_w, uint32_t *re
 int artefact_jpeg_channels(const artefact_jpeg *jpeg, artefact_channel *out_channels);
 void artefact_jpeg_free(artefact_jpeg *jpeg);
 
+/* ---- Image files: the RGB image of artefact_hip_process to file bytes ----
+ * The reference's front ends save Artefact::process's RgbImage with the `image` crate
+ * (backend/artefact-cli/main.rs:156 img.save(output); backend/artefact-wasm/lib.rs:41
+ * write_to(.., ImageFormat)), in the formats its features enable (png, bmp, tiff, webp;
+ * backend/artefact-lib/Cargo.toml:7-12).  Built here: PNG (8-bit RGB, adaptive filters,
+ * deflate), BMP (24-bit) and TIFF (baseline RGB, uncompressed).  Host code. */
+#define ARTEFACT_IMAGE_PNG  0
+#define ARTEFACT_IMAGE_BMP  1
+#define ARTEFACT_IMAGE_TIFF 2
+/* Upper bound of the encoded size of a w x h RGB image, 0 for an unknown format or empty image. */
+size_t artefact_image_encode_bound(uint32_t w, uint32_t h, uint32_t format);
+/* Encodes w*h*3 bytes of row-major interleaved RGB into out (cap bytes, at least the bound)
+ * and writes the file's size to *out_len.  ARTEFACT_E_ARGUMENT for a null pointer, an empty
+ * image, an unknown format, a file past the format's 4 GiB offsets or too small a buffer. */
+int artefact_image_encode(const uint8_t *rgb, uint32_t w, uint32_t h, uint32_t format, uint8_t *out, size_t cap,
+                          size_t *out_len);
+
 /* ---- No-allocation form of artefact_hip_compute ----
  * No counterpart in the reference.  For callers that cannot allocate device memory: code under
  * HIP graph capture, tight serving loops, hosts that forbid allocation after start-up.  The
