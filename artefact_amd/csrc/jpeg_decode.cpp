@@ -389,25 +389,43 @@ struct Decoder {
         }
     }
 
-    // The fork's restart handling: every `restart_interval` MCUs the predictions, the end-of-
-    // band run and the bit buffer restart at the next RSTn marker.
-    void restart(BitReader &br)
+    // Restart: the bit buffer, the pending marker, the end-of-band run and the DC predictions
+    // start over (bitstream.rs:669-675, mcu.rs:339-360).
+    void reset_at_marker(BitReader &br)
     {
         br.reset_bits();
-        if (br.marker >= 0xD0 && br.marker <= 0xD7) {
-            br.marker = -1;
-        } else if (br.marker < 0) {
-            // marker not reached yet: skip to it
-            while (br.pos < n) {
-                if (d[br.pos] == 0xFF && br.pos + 1 < n && d[br.pos + 1] >= 0xD0 && d[br.pos + 1] <= 0xD7) {
-                    br.pos += 2;
-                    break;
-                }
-                br.pos++;
-            }
-        }
+        br.marker = -1;
         for (Component &c : comp) c.dc_pred = 0;
         eobrun = 0;
+    }
+    static bool is_rst(int m) { return m >= 0xD0 && m <= 0xD7; }
+
+    // Sequential scans, after every MCU (mcu.rs:288-330): the countdown stops at 0, and the
+    // restart happens at the end of the first MCU at which the countdown is 0 and the bit
+    // reader has met an RSTn marker.  In a valid file that is the interval's last MCU; when the
+    // marker is missing, decoding continues without a restart until the next one is met.  With
+    // no restart interval the countdown is always 0, so a stray RSTn restarts the stream.
+    void after_sequential_mcu(BitReader &br, int &todo)
+    {
+        if (todo > 0) todo--;
+        if (todo == 0 && is_rst(br.marker)) {
+            reset_at_marker(br);
+            todo = restart_interval;
+        }
+    }
+
+    // Progressive scans, after every MCU or block (mcu_prog.rs:323-328, 409-414): at the end of
+    // every interval the countdown starts again, and the restart happens only if the reader has
+    // met an RSTn marker by then (mcu.rs:339-360); a missing marker is not searched for.  The
+    // countdown starts with each scan, as T.81 defines restart intervals (the fork carries it
+    // over from the previous scan and rejects a header marker met at an interval's end; with
+    // lazy refills, whether it has met one there depends on its bit buffer, which this decoder
+    // does not model: a scan ends with its data here).
+    void after_progressive_unit(BitReader &br, int &todo)
+    {
+        if (!restart_interval || --todo != 0) return;
+        todo = restart_interval;
+        if (is_rst(br.marker)) reset_at_marker(br);
     }
 
     void decode_sequential(BitReader &br)
@@ -439,10 +457,7 @@ struct Decoder {
                         decode_block_sequential(br, c, &c.coef[idx[b] * 64]);
                     }
                 }
-                if (restart_interval && --todo == 0) {
-                    restart(br);
-                    todo = restart_interval;
-                }
+                after_sequential_mcu(br, todo);
             }
         }
     }
@@ -539,12 +554,7 @@ struct Decoder {
     void decode_progressive_scan(BitReader &br)
     {
         int todo = restart_interval;
-        auto tick = [&]() {
-            if (restart_interval && --todo == 0) {
-                restart(br);
-                todo = restart_interval;
-            }
-        };
+        auto tick = [&]() { after_progressive_unit(br, todo); };
         if (scomp.size() == 1) {
             // One component: its blocks in raster order (mcu_prog.rs:236-330).
             Component &c = comp[scomp[0]];

@@ -56,10 +56,12 @@ def _segment(marker: int, payload: bytes) -> bytes:
     return struct.pack(">HH", 0xFF00 | marker, len(payload) + 2) + payload
 
 
-def write_baseline(width: int, height: int, components, restart_interval: int = 0) -> bytes:
+def write_baseline(width: int, height: int, components, restart_interval: int = 0, rst_log=None) -> bytes:
     """components: list of (h, v, quant[64] natural, coefs[block_h*block_w*64] natural, block_w).
 
     The component block grids must be the MCU-rounded planes (as the decoder produces them).
+    rst_log, a list, receives for every RSTn marker written (offset of its 0xFF byte in the
+    file, number of padding bits before it).
     """
     hmax = max(c[0] for c in components)
     vmax = max(c[1] for c in components)
@@ -124,7 +126,10 @@ def write_baseline(width: int, height: int, components, restart_interval: int = 
                 todo -= 1
                 last = r == mh - 1 and col == mw - 1
                 if todo == 0 and not last:
+                    pad = (8 - bits.n) % 8
                     bits.flush()
+                    if rst_log is not None:
+                        rst_log.append((len(out) + len(bits.out), pad))
                     bits.out += bytes([0xFF, 0xD0 + rst])
                     rst = (rst + 1) % 8
                     pred = [0] * len(components)

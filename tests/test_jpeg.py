@@ -290,3 +290,28 @@ def test_ac_run_past_63_lands_where_the_fork_puts_it(progressive, size, where):
     want[ZIGZAG[61]] = 1
     want[ZIGZAG[77 & 63] if where == "wrap" else ZIGZAG[63]] = size  # value 2 (2 bits) or 1 (1 bit)
     assert np.array_equal(blk, want)
+
+
+def test_missing_restart_marker_follows_the_fork():
+    """A sequential file whose restart interval is one MCU loses one RSTn marker that follows an
+    interval ending on a byte boundary (no padding bits), so the next interval's bits follow
+    seamlessly.  The fork restarts only at an MCU end where its countdown is 0 and the reader
+    has met an RSTn marker (mcu.rs:288-330): the next interval then decodes with the DC
+    predictions carried over (each component's DC values offset by its prediction at the missing
+    marker), and the interval after it, whose marker is present, decodes exactly again."""
+    w, h = 64, 48
+    coefs = synth.encode(synth.synth_rgb(h, w, 21), subsampling="444", quality=90)
+    comps = _std_components(coefs, [(1, 1)] * 3)
+    log = []
+    data = write_baseline(w, h, comps, restart_interval=1, rst_log=log)
+    k = next(i for i, (_, pad) in enumerate(log) if pad == 0 and i + 1 < len(log))
+    at = log[k][0]
+    assert data[at] == 0xFF and 0xD0 <= data[at + 1] <= 0xD7
+    cut = data[:at] + data[at + 2:]
+    full = artefact_amd.decode_jpeg(data)
+    got = artefact_amd.decode_jpeg(cut)
+    # With a one-MCU interval, MCU k + 1 is the interval after the missing marker.
+    for c in range(3):
+        want = full.coefs[c].dct_coefs.reshape(-1, 64).astype(np.int32).copy()
+        want[k + 1, 0] += want[k, 0]  # the prediction at the missing marker: the last DC before it
+        assert np.array_equal(got.coefs[c].dct_coefs.reshape(-1, 64).astype(np.int32), want), c
