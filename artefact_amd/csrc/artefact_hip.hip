@@ -1284,6 +1284,32 @@ __global__ __launch_bounds__(NT) void k_init_upsample(const ChanDev ch, const fl
     xp[i] = v;
 }
 
+// Bilinear upsampling of a component-resolution plane onto the grid, for the learned start
+// image: component sample i sits at grid coordinate (i + 0.5) * s - 0.5, and the edges clamp.
+// Without subsampling (s = 1) every weight is 0 or 1 and the plane is copied.  Replication
+// (k_init_upsample) would leave a TV jump every s pixels in a smooth prediction.
+__global__ __launch_bounds__(NT) void k_upsample_linear(const ChanDev ch, const float *plane, float *out,
+                                                         uint32_t W, uint32_t H)
+{
+    const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (i >= (size_t)W * H) return;
+    const uint32_t y = i / W, xx = i % W;
+    const float u = ((float)xx + 0.5f) / (float)ch.hs - 0.5f;
+    const float v = ((float)y + 0.5f) / (float)ch.vs - 0.5f;
+    const float fu = floorf(u), fv = floorf(v);
+    const float au = u - fu, av = v - fv;
+    auto clamp = [](float t, uint32_t n) {
+        const int k = (int)t;
+        return (uint32_t)(k < 0 ? 0 : (k > (int)n - 1 ? (int)n - 1 : k));
+    };
+    const uint32_t x0 = clamp(fu, ch.rw), x1 = clamp(fu + 1.0f, ch.rw);
+    const uint32_t y0 = clamp(fv, ch.rh), y1 = clamp(fv + 1.0f, ch.rh);
+    const float *r0 = plane + (size_t)y0 * ch.rw, *r1 = plane + (size_t)y1 * ch.rw;
+    const float top = au == 0.0f ? r0[x0] : (1.0f - au) * r0[x0] + au * r0[x1];
+    const float bot = au == 0.0f ? r1[x0] : (1.0f - au) * r1[x0] + au * r1[x1];
+    out[i] = av == 0.0f ? top : (1.0f - av) * top + av * bot;
+}
+
 // cos = c * q (coef.rs:94-104); the prob term of the first iteration is 0 (cos == c q).
 __global__ __launch_bounds__(NT) void k_init_cos(const ChanDev ch)
 {
@@ -1989,7 +2015,7 @@ struct artefact_hip_plan {
     int fuse_norm = 0;     // 1: the projection workgroups reduce the norms (no k_finalize launch)
     uint32_t lane_units[3] = {0, 0, 0};  // k_project_lane: cumulative units (one wave's blocks) per channel
     uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
-    float qmean[3] = {0, 0, 0};                   // mean quantisation step per channel (warm start)
+    float qtab[3][64] = {};                       // quantisation tables per channel, f32 (warm-start features)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
     ChanDev ch[3];
@@ -2197,7 +2223,9 @@ int launch_iterations(artefact_hip_plan *pl, hipStream_t s)
 }
 
 // Start image from the coefficients (no override): the kernels of artefact_hip_plan_init.
-void launch_init(artefact_hip_plan *pl, hipStream_t s)
+// upsample = false (the learned start): only the component planes and cos / prob; x and
+// x_prev are left for the caller to write.
+void launch_init(artefact_hip_plan *pl, hipStream_t s, bool upsample = true)
 {
     for (uint32_t c = 0; c < pl->C; c++) {
         const ChanDev &d = pl->ch[c];
@@ -2205,8 +2233,9 @@ void launch_init(artefact_hip_plan *pl, hipStream_t s)
         float *plane = pl->g + c * pl->N;  // scratch: M_c <= N
         const size_t nb = (size_t)d.bw * d.bh;
         hipLaunchKernelGGL(k_init_blocks, dim3((unsigned)((nb + NT / 8 - 1) / (NT / 8))), dim3(NT), 0, s, d, plane);
-        hipLaunchKernelGGL(k_init_upsample, dim3((unsigned)((pl->N + NT - 1) / NT)), dim3(NT), 0, s, d,
-                           (const float *)plane, x, xp, pl->W, pl->H);
+        if (upsample)
+            hipLaunchKernelGGL(k_init_upsample, dim3((unsigned)((pl->N + NT - 1) / NT)), dim3(NT), 0, s, d,
+                               (const float *)plane, x, xp, pl->W, pl->H);
         hipLaunchKernelGGL(k_init_cos, dim3((unsigned)((pl->M[c] + NT - 1) / NT)), dim3(NT), 0, s, d);
     }
 }
@@ -2321,14 +2350,12 @@ void set_loop_scalars(artefact_hip_plan *pl, const artefact_params *p)
 // correctly rounded f64 reciprocal of q*q, the product rounded to f32 first (compute_step_prob.rs:29).
 void set_channel(artefact_hip_plan *pl, uint32_t c, const artefact_channel &k, float *quant, double *rqq)
 {
-    double qsum = 0.0;
     for (int j = 0; j < 64; j++) {
         const float q = (float)k.quant[j];
         quant[j] = q;
         rqq[j] = 1.0 / (double)(q * q);
-        qsum += q;
+        pl->qtab[c][j] = q;
     }
-    pl->qmean[c] = (float)(qsum / 64.0);
     ChanDev &d = pl->ch[c];
     d.rw = k.rounded_w;
     d.rh = k.rounded_h;
@@ -2656,6 +2683,9 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     const size_t at_quant = take(3 * 64 * 4), at_rqq = take(3 * 64 * 8), at_part = take(nwg * 6 * 8);
     const size_t at_norms = take(3 * 4), at_rnorm = take(3 * 8), at_nsq = take(3 * 8);
     ok = ok && hipMalloc(&pl->arena, arena_bytes) == hipSuccess;
+    // A failed allocation stays the thread's last HIP error until read; it is handled here, so it
+    // must not surface as a later launch's error.
+    if (!ok) (void)hipGetLastError();
     if (ok) {
         char *base = static_cast<char *>(pl->arena);
         pl->buf[0] = reinterpret_cast<float *>(base + at_x);
@@ -2935,6 +2965,7 @@ int artefact_hip_plan_objective(artefact_hip_plan *pl, double *terms)
     double *part = nullptr, *out = nullptr;
     bool ok = hipMalloc(&part, ((size_t)pix_wgs + blk_wgs) * 6 * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&out, 3 * sizeof(double)) == hipSuccess;
+    if (!ok) (void)hipGetLastError();  // handled here: not a later launch's error
     if (ok) {
         a.partials = part;
         a.pixel_wgs = pix_wgs;
@@ -3077,15 +3108,37 @@ int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
     DeviceGuard guard(pl->device);
     // The buffers are overwritten from here on: a failure below leaves no start image.
     pl->initialized = false;
-    launch_init(pl, pl->stream);
-    const float *planes[3];
-    float *pred[3];
-    for (uint32_t c = 0; c < pl->C; c++) {
-        planes[c] = pl->buf[0] + (size_t)c * pl->N;
-        pred[c] = pl->ybuf + (size_t)c * pl->N;
+    // The decode: each channel's component plane (rw x rh) in the gradient buffer, and cos /
+    // prob; x and x_prev are written below, from the prediction.
+    launch_init(pl, pl->stream, false);
+    // The CNN runs on every channel at its component resolution, as the decoder produced it
+    // (subsampled chroma at a quarter of the grid's pixels), channels of equal size in one call.
+    // Its prediction goes to the channel's prob plane (M_c floats, free until the projection
+    // below writes it) and is upsampled bilinearly onto the grid, into the y buffer the
+    // projection reads.
+    bool done[3] = {false, false, false};
+    for (uint32_t c0 = 0; c0 < pl->C; c0++) {
+        if (done[c0]) continue;
+        const float *planes[3];
+        float *pred[3];
+        float quant[3 * 64];
+        uint32_t n = 0;
+        for (uint32_t c = c0; c < pl->C; c++) {
+            if (done[c] || pl->ch[c].rw != pl->ch[c0].rw || pl->ch[c].rh != pl->ch[c0].rh) continue;
+            planes[n] = pl->g + (size_t)c * pl->N;
+            pred[n] = pl->ch[c].prob;
+            memcpy(quant + n * 64, pl->qtab[c], 64 * sizeof(float));
+            done[c] = true;
+            n++;
+        }
+        const int rc = cnn_predict_planes(net, planes, quant, n, pl->ch[c0].rh, pl->ch[c0].rw, pred, pl->stream);
+        if (rc) return rc;
     }
-    int rc = cnn_predict_planes(net, planes, pl->qmean, pl->C, pl->H, pl->W, pred, pl->stream);
-    if (rc) return rc;
+    for (uint32_t c = 0; c < pl->C; c++) {
+        float *y = pl->ybuf + (size_t)c * pl->N;
+        hipLaunchKernelGGL(k_upsample_linear, dim3((unsigned)((pl->N + NT - 1) / NT)), dim3(NT), 0, pl->stream,
+                           pl->ch[c], (const float *)pl->ch[c].prob, y, pl->W, pl->H);
+    }
     // Projection of the prediction: k_project with zero norms (no descent step) reads y =
     // prediction and writes the projected image into buf[1] and the prob plane of its clamped
     // coefficients, the state the loop carries into its next gradient.

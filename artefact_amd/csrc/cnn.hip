@@ -3,8 +3,10 @@
 //
 // Network: five 3x3 convolutions with zero padding, widths 2 -> 32 -> 32 -> 32 -> 32 -> 1,
 // ReLU after the first four, the output added to input feature 0.  Features of a plane:
-// f0 = x0 / 64 (the decoded, box-projected plane) and f1 = qstep / 64 (the channel's mean
-// quantisation step); the prediction is 64 * (f0 + net(f)).  Activations are NHWC bf16 with
+// f0 = x0 / 64 (the decoded, box-projected plane) and f1 = q(y mod 8, x mod 8) / 64, the
+// plane's quantisation table tiled over its 8x8 blocks (a per-pixel quantisation-step map, which
+// tells the network each pixel's position in its block as well as the quantisation strength);
+// the prediction is 64 * (f0 + net(f)).  Activations are NHWC bf16 with
 // the channel dimension padded to 16 or 32; weights are kept in f32 (Adam's master copy) and
 // repacked to bf16 operand layouts after every update.
 //
@@ -79,6 +81,13 @@ constexpr ParamLayout PL = param_layout();
 // k_pack_input stores) and the prediction written in image units (x 64), for inference.
 enum { MODE_RELU = 0, MODE_LINEAR = 1, MODE_FINAL = 2, MODE_FINAL_IMAGE = 3 };
 
+// A plane's quantisation table (natural order), passed by value to the kernels that form the
+// features; pixel (y, x) of the plane takes entry (y mod 8) * 8 + (x mod 8).
+struct QuantTable {
+    float q[64];
+};
+__device__ __forceinline__ int qmap_index(int y, int x) { return ((y & 7) << 3) | (x & 7); }
+
 // Output tile of one workgroup: 64 pixels (wave w: pixels 16w .. 16w+15) x RT rows.  The input
 // tile (RT + 2 rows x 66 pixels x CI channels, zero outside the plane) is staged in LDS once;
 // each wave walks its input rows and feeds every B operand it reads to the (up to three)
@@ -91,13 +100,13 @@ template <int CI>
 constexpr int lds_px_bytes() { return CI * 2 + 8; }  // padded pixel stride (bank spread), a multiple of 8
 
 // FEAT_IN: the input is the decoded plane itself (f32, `in` cast back) and the kernel forms the
-// 16 feature channels while staging its tile: bf16(x0 / 64), bf16(qstep / 64), then zeros, as
-// k_pack_input does.
+// 16 feature channels while staging its tile: bf16(x0 / 64), bf16(q(y mod 8, x mod 8) / 64), then
+// zeros, as k_pack_input does.
 template <int CI, int CO, int MODE, bool FEAT_IN = false>
 __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, const bf16x4 *__restrict__ wpk,
                                              const float *__restrict__ bias, uint16_t *__restrict__ out,
                                              float *__restrict__ pred, const float *__restrict__ resid, int H,
-                                             int W, float qstep)
+                                             int W, const QuantTable qt)
 {
     static_assert(!FEAT_IN || CI == 16, "features fill one 16-channel chunk");
     constexpr int KC = CI / 16, MH = CO / 16, PXB = lds_px_bytes<CI>();
@@ -119,7 +128,8 @@ __global__ __launch_bounds__(CT) void k_conv(const uint16_t *__restrict__ in, co
         uint4 v;
         if constexpr (FEAT_IN) {
             const float x0 = reinterpret_cast<const float *>(in)[pix];
-            const uint32_t f01 = (uint32_t)f2bf(x0 * FEAT) | ((uint32_t)f2bf(qstep * FEAT) << 16);
+            const uint32_t f01 =
+                (uint32_t)f2bf(x0 * FEAT) | ((uint32_t)f2bf(qt.q[qmap_index(gy, gx)] * FEAT) << 16);
             v = make_uint4(part == 0 ? f01 : 0u, 0u, 0u, 0u);
         } else {
             v = *reinterpret_cast<const uint4 *>(in + pix * CI + part * 8);
@@ -331,18 +341,20 @@ __global__ __launch_bounds__(CT) void k_pack_weights(const float *w, int cin, in
     dst[i] = v;
 }
 
-// Features of B planes: channel 0 = x0 / 64, channel 1 = qstep / 64, channels 2..15 = 0.
-__global__ __launch_bounds__(CT) void k_pack_input(const float *x0, const float *qstep, size_t plane, int B,
+// Features of B planes of width w: channel 0 = x0 / 64, channel 1 = q(y mod 8, x mod 8) / 64 with
+// plane b's table quant[b * 64 ..], channels 2..15 = 0.
+__global__ __launch_bounds__(CT) void k_pack_input(const float *x0, const float *quant, size_t plane, int w, int B,
                                                    uint16_t *a0, float *resid)
 {
     const size_t i = (size_t)blockIdx.x * CT + threadIdx.x;
     if (i >= plane * B) return;
     const int b = (int)(i / plane);
+    const int p = (int)(i - (size_t)b * plane);
     const float f0 = x0[i] * FEAT;
     resid[i] = f0;
     uint16_t *o = a0 + i * 16;
     o[0] = f2bf(f0);
-    o[1] = f2bf(qstep[b] * FEAT);
+    o[1] = f2bf(quant[b * 64 + qmap_index(p / w, p % w)] * FEAT);
 #pragma unroll
     for (int c = 2; c < 16; c++) o[c] = 0;
 }
@@ -429,7 +441,8 @@ constexpr int SSTR = 80;                          // bytes per ring pixel: 32 bf
 constexpr int SROW = SPX * SSTR;                  // one ring row
 constexpr int SRING = 2 * SROW;                   // one activation ring (rows alternate by step)
 constexpr int SFEAT = 4 * SPX * 4;                // feature ring: 4 rows of (f0, q) bf16 pairs
-constexpr int SLDS = 4 * SRING + SFEAT;           // rings of layers 1-4, then the features
+constexpr int SQMAP = 64 * 4;                     // the plane's q map entries, bf16 in the high half
+constexpr int SLDS = 4 * SRING + SFEAT + SQMAP;   // rings of layers 1-4, the features, the q map
 constexpr int SFILL = 12;                         // steps from a band's first input row to its first output row
 // Packed weights, one u32x4 (8 bf16) per lane: layer 1 [mh][lane]; layers 2-4 [tap][mh][lane];
 // layer 5 [tap][lane].
@@ -511,7 +524,7 @@ __device__ __forceinline__ void stream_mid(uint8_t *lds, const u32x4 *__restrict
 // Layer 1 and the feature staging: one wave.
 __device__ __forceinline__ void stream_first(uint8_t *lds, const float *__restrict__ x0, const u32x4 *__restrict__ wts,
                                              const float *__restrict__ params, int bx0, int y0, int H, int W, int T,
-                                             float qstep, int lane)
+                                             const float *__restrict__ quant, int lane)
 {
     const int n = lane & 15, g = lane >> 4;
     u32x4 w1[2];
@@ -523,8 +536,9 @@ __device__ __forceinline__ void stream_first(uint8_t *lds, const float *__restri
 #pragma unroll
         for (int k = 0; k < 4; k++) b1[mh][k] = params[PL.b[0] + mh * 16 + 4 * g + k];
     uint32_t *feat = reinterpret_cast<uint32_t *>(lds + 4 * SRING);  // [4][SPX]
+    uint32_t *qmap = reinterpret_cast<uint32_t *>(lds + 4 * SRING + SFEAT);
     uint8_t *r1 = lds;                                               // layer 1's output ring
-    const uint32_t qbits = (uint32_t)f2bf(qstep * FEAT) << 16;
+    qmap[lane] = (uint32_t)f2bf(quant[lane] * FEAT) << 16;  // read only by this wave
 
     // Feature row f is plane row y0 - 5 + f; ring pixel p is plane column bx0 - 5 + p.
     auto load_row = [&](int f, float v[2]) {
@@ -542,7 +556,7 @@ __device__ __forceinline__ void stream_first(uint8_t *lds, const float *__restri
             const int p = lane + 64 * k;
             const int x = bx0 - 5 + p;
             const bool ok = y >= 0 && y < H && x >= 0 && x < W;
-            if (p < SPX) feat[(f & 3) * SPX + p] = ok ? ((uint32_t)f2bf(v[k] * FEAT) | qbits) : 0u;
+            if (p < SPX) feat[(f & 3) * SPX + p] = ok ? ((uint32_t)f2bf(v[k] * FEAT) | qmap[qmap_index(y, x)]) : 0u;
         }
     };
     float pf[2];
@@ -651,7 +665,7 @@ constexpr int SPLANES = 4;
 struct StreamPlanes {
     const float *x0[SPLANES];
     float *out[SPLANES];
-    float qstep[SPLANES];
+    float quant[SPLANES][64];
 };
 
 __global__ __launch_bounds__(SCT, 4) void k_cnn_stream(const StreamPlanes pp, const u32x4 *__restrict__ wts,
@@ -659,14 +673,14 @@ __global__ __launch_bounds__(SCT, 4) void k_cnn_stream(const StreamPlanes pp, co
 {
     const float *__restrict__ x0 = pp.x0[blockIdx.z];
     float *__restrict__ out = pp.out[blockIdx.z];
-    const float qstep = pp.qstep[blockIdx.z];
+    const float *quant = pp.quant[blockIdx.z];
     __shared__ __attribute__((aligned(16))) uint8_t lds[SLDS];
     for (int i = threadIdx.x; i < SLDS / 16; i += SCT) reinterpret_cast<u32x4 *>(lds)[i] = u32x4{0u, 0u, 0u, 0u};
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int bx0 = blockIdx.x * SB, y0 = blockIdx.y * RH, T = RH + SFILL;
-    if (wave == 0) stream_first(lds, x0, wts, params, bx0, y0, H, W, T, qstep, lane);
+    if (wave == 0) stream_first(lds, x0, wts, params, bx0, y0, H, W, T, quant, lane);
     else if (wave == 7) stream_last(lds, x0, wts, params, out, bx0, y0, RH, H, W, T, lane);
     else stream_mid(lds, wts, params, 2 + (wave - 1) / 2, (wave - 1) & 1, bx0, y0, H, W, T, lane);
 }
@@ -715,7 +729,8 @@ struct artefact_cnn {
     size_t cap_px = 0;
     uint16_t *act[NLAYER] = {};  // act[0] = features (16 ch), act[l] = output of layer l-1 (32 ch)
     uint16_t *g32a = nullptr, *g32b = nullptr, *g16 = nullptr;
-    float *resid = nullptr, *pred = nullptr, *x0 = nullptr, *target = nullptr, *qstep = nullptr;
+    float *resid = nullptr, *pred = nullptr, *x0 = nullptr, *target = nullptr;
+    float *quant = nullptr;  // [64 planes][64]: the tables of a batch
     float *part = nullptr;
     double *lpart = nullptr;
     int wgroups = 0;
@@ -755,6 +770,9 @@ bool ensure_workspace(artefact_cnn *net, size_t px)
     ok = ok && hipMalloc(&net->target, px * 4) == hipSuccess;
     ok = ok && hipMalloc(&net->lpart, blocks(px) * sizeof(double)) == hipSuccess;
     net->cap_px = ok ? px : 0;
+    // A failed allocation leaves its error as the thread's last HIP error, which the next launch
+    // check (hipGetLastError) would report as its own: clear it here, where it is handled.
+    if (!ok) (void)hipGetLastError();
     return ok;
 }
 
@@ -772,10 +790,10 @@ void repack(artefact_cnn *net)
 
 template <int CI, int CO, int MODE, bool FEAT_IN = false>
 void conv(const uint16_t *in, const bf16x4 *w, const float *bias, uint16_t *out, float *pred,
-          const float *resid, int B, int H, int W, hipStream_t s, float qstep = 0.0f)
+          const float *resid, int B, int H, int W, hipStream_t s, const QuantTable &qt = QuantTable{})
 {
     hipLaunchKernelGGL((k_conv<CI, CO, MODE, FEAT_IN>), dim3((W + 63) / 64, (H + RT - 1) / RT, B), dim3(CT), 0, s,
-                       in, w, bias, out, pred, resid, H, W, qstep);
+                       in, w, bias, out, pred, resid, H, W, qt);
 }
 
 // Forward pass on the features in act[0] / resid; the prediction (feature units) in pred.
@@ -841,10 +859,10 @@ bool upload_batch(artefact_cnn *net, const float *x0, const float *q, const floa
     if (!ensure_workspace(net, px)) return false;
     const hipMemcpyKind kind = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     bool ok = hipMemcpyAsync(net->x0, x0, px * 4, kind, net->stream) == hipSuccess;
-    ok = ok && hipMemcpyAsync(net->qstep, q, (size_t)B * 4, kind, net->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(net->quant, q, (size_t)B * 64 * 4, kind, net->stream) == hipSuccess;
     if (target) ok = ok && hipMemcpyAsync(net->target, target, px * 4, kind, net->stream) == hipSuccess;
     hipLaunchKernelGGL(k_pack_input, dim3(blocks(px)), dim3(CT), 0, net->stream, (const float *)net->x0,
-                       (const float *)net->qstep, (size_t)H * W, B, net->act[0], net->resid);
+                       (const float *)net->quant, (size_t)H * W, W, B, net->act[0], net->resid);
     return ok && hipGetLastError() == hipSuccess;
 }
 
@@ -858,10 +876,10 @@ bool layered_requested()
 
 }  // namespace
 
-int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const float *qsteps, uint32_t nplanes,
+int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const float *quant, uint32_t nplanes,
                        uint32_t h, uint32_t w, float *const *out, hipStream_t s)
 {
-    if (!net || !planes || !qsteps || !out || !nplanes || nplanes > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
+    if (!net || !planes || !quant || !out || !nplanes || nplanes > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
     // A training step returns with its Adam update and weight repacking still queued on the
     // net's stream; the kernels below read those weights on stream s.
     if (hipStreamSynchronize(net->stream) != hipSuccess) return ARTEFACT_E_HIP;
@@ -885,7 +903,7 @@ int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const floa
             for (uint32_t k = 0; k < n; k++) {
                 pp.x0[k] = planes[i0 + k];
                 pp.out[k] = out[i0 + k];
-                pp.qstep[k] = qsteps[i0 + k];
+                memcpy(pp.quant[k], quant + (size_t)(i0 + k) * 64, 64 * sizeof(float));
             }
             hipLaunchKernelGGL(k_cnn_stream, dim3(nbx, nby, n), dim3(SCT), 0, s, pp, (const u32x4 *)net->ws,
                                (const float *)net->params, (int)h, (int)w, rh);
@@ -898,9 +916,11 @@ int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const floa
     // The first layer forms the features from the plane while staging its tiles and the last
     // writes the prediction in image units, so no packing kernels run (same arithmetic).
     for (uint32_t i = 0; i < nplanes && ok; i++) {
+        QuantTable qt;
+        memcpy(qt.q, quant + (size_t)i * 64, sizeof qt.q);
         conv<16, 32, MODE_RELU, true>(reinterpret_cast<const uint16_t *>(planes[i]), net->wf[0],
                                       net->params + PL.b[0], net->act[1], nullptr, nullptr, 1, (int)h, (int)w, s,
-                                      qsteps[i]);
+                                      qt);
         conv<32, 32, MODE_RELU>(net->act[1], net->wf[1], net->params + PL.b[1], net->act[2], nullptr, nullptr, 1,
                                 (int)h, (int)w, s);
         conv<32, 32, MODE_RELU>(net->act[2], net->wf[2], net->params + PL.b[2], net->act[3], nullptr, nullptr, 1,
@@ -929,7 +949,7 @@ int artefact_cnn_create(const float *params, artefact_cnn **out)
     const size_t pb = (size_t)PL.total * 4;
     ok = ok && hipMalloc(&net->params, pb) == hipSuccess && hipMalloc(&net->grads, pb) == hipSuccess;
     ok = ok && hipMalloc(&net->m, pb) == hipSuccess && hipMalloc(&net->v, pb) == hipSuccess;
-    ok = ok && hipMalloc(&net->qstep, 64 * 4) == hipSuccess;
+    ok = ok && hipMalloc(&net->quant, 64 * 64 * 4) == hipSuccess;
     for (int l = 0; l < NLAYER && ok; l++) {
         const size_t nf = (size_t)9 * (CINP[l] / 16) * (COUTP[l] / 16) * 64 * sizeof(bf16x4);
         ok = hipMalloc(&net->wf[l], nf) == hipSuccess && hipMalloc(&net->wb[l], nf) == hipSuccess;
@@ -945,6 +965,7 @@ int artefact_cnn_create(const float *params, artefact_cnn **out)
     ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
     if (!ok) {
         artefact_cnn_destroy(net);
+        (void)hipGetLastError();  // handled here: not the next call's launch error
         return ARTEFACT_E_HIP;
     }
     *out = net;
@@ -959,10 +980,10 @@ int artefact_cnn_get_params(artefact_cnn *net, float *out)
     return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
 
-int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *qsteps, uint32_t batch, uint32_t h,
+int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *quant, uint32_t batch, uint32_t h,
                          uint32_t w, float *pred)
 {
-    if (!net || !x0 || !qsteps || !pred || !batch || batch > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
+    if (!net || !x0 || !quant || !pred || !batch || batch > 64 || !h || !w) return ARTEFACT_E_ARGUMENT;
     const size_t plane = (size_t)h * w, px = batch * plane;
     if (!ensure_workspace(net, px)) return ARTEFACT_E_HIP;
     // The inference path warm_init runs, plane by plane.
@@ -973,17 +994,17 @@ int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *qsteps
         outs[i] = net->pred + i * plane;
     }
     bool ok = hipMemcpyAsync(net->x0, x0, px * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
-    ok = ok && cnn_predict_planes(net, planes, qsteps, batch, h, w, outs, net->stream) == ARTEFACT_OK;
+    ok = ok && cnn_predict_planes(net, planes, quant, batch, h, w, outs, net->stream) == ARTEFACT_OK;
     ok = ok && hipMemcpyAsync(pred, net->pred, px * 4, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
     ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
     return ok ? ARTEFACT_OK : ARTEFACT_E_HIP;
 }
 
-int artefact_cnn_gradients(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+int artefact_cnn_gradients(artefact_cnn *net, const float *x0, const float *quant, const float *target,
                            uint32_t batch, uint32_t h, uint32_t w, double *loss, float *grads)
 {
-    if (!net || !x0 || !qsteps || !target || !batch || batch > 64 || !h || !w || w % 16) return ARTEFACT_E_ARGUMENT;
-    if (!upload_batch(net, x0, qsteps, target, (int)batch, (int)h, (int)w, false)) return ARTEFACT_E_HIP;
+    if (!net || !x0 || !quant || !target || !batch || batch > 64 || !h || !w || w % 16) return ARTEFACT_E_ARGUMENT;
+    if (!upload_batch(net, x0, quant, target, (int)batch, (int)h, (int)w, false)) return ARTEFACT_E_HIP;
     forward(net, (int)batch, (int)h, (int)w, net->stream);
     if (!backward(net, (int)batch, (int)h, (int)w, loss)) return ARTEFACT_E_HIP;
     if (grads) {
@@ -995,20 +1016,22 @@ int artefact_cnn_gradients(artefact_cnn *net, const float *x0, const float *qste
     return ARTEFACT_OK;
 }
 
-int artefact_cnn_set_dataset(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+int artefact_cnn_set_dataset(artefact_cnn *net, const float *x0, const float *quant, const float *target,
                              uint32_t count, uint32_t h, uint32_t w)
 {
-    if (!net || !x0 || !qsteps || !target || !count || !h || !w || w % 16) return ARTEFACT_E_ARGUMENT;
+    if (!net || !x0 || !quant || !target || !count || !h || !w || w % 16) return ARTEFACT_E_ARGUMENT;
     for (float *p : {net->ds_x0, net->ds_target, net->ds_q})
         if (p) (void)hipFree(p);
     net->ds_x0 = net->ds_target = net->ds_q = nullptr;
     const size_t px = (size_t)count * h * w;
     bool ok = hipMalloc(&net->ds_x0, px * 4) == hipSuccess && hipMalloc(&net->ds_target, px * 4) == hipSuccess &&
-              hipMalloc(&net->ds_q, (size_t)count * 4) == hipSuccess;
+              hipMalloc(&net->ds_q, (size_t)count * 64 * 4) == hipSuccess;
     ok = ok && hipMemcpyAsync(net->ds_x0, x0, px * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
     ok = ok && hipMemcpyAsync(net->ds_target, target, px * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
-    ok = ok && hipMemcpyAsync(net->ds_q, qsteps, (size_t)count * 4, hipMemcpyHostToDevice, net->stream) == hipSuccess;
+    ok = ok && hipMemcpyAsync(net->ds_q, quant, (size_t)count * 64 * 4, hipMemcpyHostToDevice, net->stream) ==
+                   hipSuccess;
     ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+    if (!ok) (void)hipGetLastError();  // handled here: not the next call's launch error
     net->ds_count = ok ? count : 0;
     net->ds_h = h;
     net->ds_w = w;
@@ -1029,13 +1052,13 @@ int artefact_cnn_train_step(artefact_cnn *net, const uint32_t *indices, uint32_t
                             hipMemcpyDeviceToDevice, net->stream) == hipSuccess;
         ok = ok && hipMemcpyAsync(net->target + i * plane, net->ds_target + indices[i] * plane, plane * 4,
                                   hipMemcpyDeviceToDevice, net->stream) == hipSuccess;
-        ok = ok && hipMemcpyAsync(net->qstep + i, net->ds_q + indices[i], 4, hipMemcpyDeviceToDevice, net->stream) ==
-                       hipSuccess;
+        ok = ok && hipMemcpyAsync(net->quant + i * 64, net->ds_q + (size_t)indices[i] * 64, 64 * 4,
+                                  hipMemcpyDeviceToDevice, net->stream) == hipSuccess;
     }
     // A failed copy leaves the batch incomplete: no step is taken on it.
     if (!ok) return ARTEFACT_E_HIP;
     hipLaunchKernelGGL(k_pack_input, dim3(blocks(plane * batch)), dim3(CT), 0, net->stream, (const float *)net->x0,
-                       (const float *)net->qstep, plane, (int)batch, net->act[0], net->resid);
+                       (const float *)net->quant, plane, W, (int)batch, net->act[0], net->resid);
     forward(net, (int)batch, H, W, net->stream);
     ok = ok && backward(net, (int)batch, H, W, loss);
     net->adam_t++;
@@ -1052,7 +1075,7 @@ void artefact_cnn_destroy(artefact_cnn *net)
 {
     if (!net) return;
     if (net->stream) (void)hipStreamSynchronize(net->stream);
-    std::vector<void *> bufs{net->params, net->grads, net->m, net->v, net->qstep, net->part, net->g32a, net->g32b,
+    std::vector<void *> bufs{net->params, net->grads, net->m, net->v, net->quant, net->part, net->g32a, net->g32b,
                              net->g16, net->resid, net->pred, net->x0, net->target, net->lpart, net->ds_x0,
                              net->ds_target, net->ds_q, net->ws};
     for (int l = 0; l < NLAYER; l++) {

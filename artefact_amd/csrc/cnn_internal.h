@@ -6,9 +6,9 @@
 
 #include "../../include/artefact_hip.h"
 
-// Prediction of `nplanes` device planes of h x w (image units) into device planes `out`,
-// with per-plane quantisation steps (host values), on stream s.
-int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const float *qsteps, uint32_t nplanes,
+// Prediction of `nplanes` device planes of h x w (image units) into device planes `out`, with
+// each plane's quantisation table (host values, 64 per plane, natural order), on stream s.
+int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const float *quant, uint32_t nplanes,
                        uint32_t h, uint32_t w, float *const *out, hipStream_t s);
 
 #endif

@@ -1,9 +1,11 @@
 """Learned warm start: a residual CNN whose prediction starts the reconstruction loop.
 
-The network (csrc/cnn.hip) maps a decoded plane and its channel's mean quantisation step to
-a prediction of the converged plane.  Training pairs come from the solver itself: the
-decoded start image of a synthetic crop (already inside the quantisation box) and the plain
-TGV loop's output after ``target_iterations`` iterations.  Forward, backward and Adam run on
+The network (csrc/cnn.hip) maps a decoded plane and its quantisation-step map (the channel's
+64-entry table tiled over its 8x8 blocks) to a prediction of the converged plane, at the
+component's own resolution.  Training pairs come from the solver itself: the decoded start
+image of a synthetic crop (already inside the quantisation box) and the plain TGV loop's output
+after ``target_iterations`` iterations (for subsampled chroma, its 2x2 means: what the
+projection keeps of a prediction replicated onto the grid).  Forward, backward and Adam run on
 the GPU.  ``warm_solve`` starts a plan from the projected prediction, runs its iterations and
 falls back to the plain loop when the warm result's objective is higher than the plain
 result's (the objective decides).  No reference counterpart.
@@ -24,6 +26,8 @@ from .compute import Plan
 _FP = ctypes.POINTER(ctypes.c_float)
 WIDTHS = (2, 32, 32, 32, 32, 1)
 WEIGHTS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
+# What the weights file's network was trained on (stored with the weights, checked on load).
+INPUT_FEATURES = "plane/64 and tiled quantisation table/64, components at their own resolution"
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -70,6 +74,8 @@ class WarmStartNet:
     @classmethod
     def load(cls, path: str = WEIGHTS_FILE) -> "WarmStartNet":
         with np.load(path, allow_pickle=False) as f:
+            if str(f.get("input_features", "")) != INPUT_FEATURES:
+                raise ValueError(f"{path}: weights trained on other input features")
             return cls(f["params"])
 
     def save(self, path: str, **meta) -> None:
@@ -80,19 +86,19 @@ class WarmStartNet:
         _lib.check(self._lib.artefact_cnn_get_params(self._h, _fp(out)), "artefact_cnn_get_params")
         return out
 
-    def forward(self, x0: np.ndarray, qsteps) -> np.ndarray:
-        """Predictions (image units) for planes x0 (B, H, W)."""
+    def forward(self, x0: np.ndarray, quant) -> np.ndarray:
+        """Predictions (image units) for planes x0 (B, H, W) with quantisation tables (B, 64)."""
         x = np.ascontiguousarray(x0, dtype=np.float32)
-        q = np.ascontiguousarray(qsteps, dtype=np.float32)
+        q = _tables(quant, x.shape[0])
         out = np.empty_like(x)
         B, H, W = x.shape
         _lib.check(self._lib.artefact_cnn_forward(self._h, _fp(x), _fp(q), B, H, W, _fp(out)), "artefact_cnn_forward")
         return out
 
-    def gradients(self, x0, qsteps, target):
+    def gradients(self, x0, quant, target):
         """(loss, flat parameter gradients) of the mean squared error in units of 1/64."""
         x = np.ascontiguousarray(x0, dtype=np.float32)
-        q = np.ascontiguousarray(qsteps, dtype=np.float32)
+        q = _tables(quant, x.shape[0])
         t = np.ascontiguousarray(target, dtype=np.float32)
         g = np.empty(self.n, dtype=np.float32)
         loss = ctypes.c_double()
@@ -101,9 +107,9 @@ class WarmStartNet:
                                                     _fp(g)), "artefact_cnn_gradients")
         return loss.value, g
 
-    def set_dataset(self, x0, qsteps, target) -> None:
+    def set_dataset(self, x0, quant, target) -> None:
         x = np.ascontiguousarray(x0, dtype=np.float32)
-        q = np.ascontiguousarray(qsteps, dtype=np.float32)
+        q = _tables(quant, x.shape[0])
         t = np.ascontiguousarray(target, dtype=np.float32)
         B, H, W = x.shape
         _lib.check(self._lib.artefact_cnn_set_dataset(self._h, _fp(x), _fp(q), _fp(t), B, H, W),
@@ -130,8 +136,16 @@ class WarmStartNet:
             pass
 
 
-def qstep(coef) -> float:
-    return float(np.asarray(coef.quant_table, dtype=np.float64).mean())
+def _tables(quant, n):
+    q = np.ascontiguousarray(quant, dtype=np.float32).reshape(-1, 64)
+    if q.shape[0] != n:
+        raise ValueError(f"expected {n} quantisation tables of 64 entries, got {q.shape[0]}")
+    return q
+
+
+def quant_table(coef) -> np.ndarray:
+    """The component's quantisation table (natural order) as the network's feature input."""
+    return np.asarray(coef.quant_table, dtype=np.float32).reshape(64)
 
 
 def _encode_crop(job):
@@ -140,21 +154,33 @@ def _encode_crop(job):
     return synth.encode(synth.synth_rgb(size, size, img_seed), subsampling="420", quality=quality)
 
 
+SAMPLES_PER_CROP = 6  # four luma quarters and the two chroma components of a 2 size x 2 size crop
+
+
 def training_pairs(count: int, size: int = 256, seed: int = 0, target_iterations: int = 300,
                    qualities=(20, 90), weight: float = 0.3, pweight: float = 0.001, log=print,
                    workers: int = 8):
-    """``count`` synthetic 4:2:0 crops of size x size at qualities drawn uniformly in the
-    range; every channel plane (grid resolution) is one sample.  Returns (x0, q, target) with
-    x0 the decoded start plane and target the plain TGV loop's output, all float32.  The
-    crops are synthesised and encoded by ``workers`` spawned processes (host work only); the
-    start images and targets are solved on the GPU by this process."""
+    """``count`` synthetic 4:2:0 crops of 2 size x 2 size at qualities drawn uniformly in the
+    range, each giving six samples of size x size at the resolution the network sees in
+    inference: the four quarters of the luma plane, and each chroma component at its own
+    (half) resolution.  Returns (x0, quant, target): x0 the decoded start plane (a chroma
+    component before its upsampling), quant its quantisation table (per sample, 64 entries),
+    target the plain TGV loop's output (for chroma, the 2x2 means of the solved grid plane), all
+    float32.  The crops are synthesised and encoded by ``workers`` spawned processes (host work
+    only); the start images and targets are solved on the GPU by this process."""
     import multiprocessing
 
     rng = np.random.default_rng(seed)
-    jobs = [(size, seed * 100003 + i, int(rng.integers(qualities[0], qualities[1] + 1))) for i in range(count)]
-    x0 = np.empty((3 * count, size, size), np.float32)
+    big = 2 * size
+    jobs = [(big, seed * 100003 + i, int(rng.integers(qualities[0], qualities[1] + 1))) for i in range(count)]
+    n = SAMPLES_PER_CROP * count
+    x0 = np.empty((n, size, size), np.float32)
     target = np.empty_like(x0)
-    qs = np.empty(3 * count, np.float32)
+    qs = np.empty((n, 64), np.float32)
+
+    def quarters(p):
+        return [p[:size, :size], p[:size, size:], p[size:, :size], p[size:, size:]]
+
     chunk = 64  # graphs of at most 64 images
     ctx = multiprocessing.get_context("spawn")
     with ctx.Pool(max(1, workers)) as pool:
@@ -163,18 +189,27 @@ def training_pairs(count: int, size: int = 256, seed: int = 0, target_iterations
             coefs_list = [next(encoded) for _ in range(min(chunk, count - c0))]
             imgs = []
             for k, coefs in enumerate(coefs_list):
-                i = c0 + k
-                imgs.append(BatchImage(coefs, weight, [pweight] * 3, target_iterations, size, size, "tgv"))
-                with Plan(coefs, weight, [pweight] * 3, 0, size, size) as plan:
+                i = SAMPLES_PER_CROP * (c0 + k)
+                imgs.append(BatchImage(coefs, weight, [pweight] * 3, target_iterations, big, big, "tgv"))
+                with Plan(coefs, weight, [pweight] * 3, 0, big, big) as plan:
                     plan.init()
-                    x0[3 * i:3 * i + 3] = np.stack(plan.download()).reshape(3, size, size)
-                qs[3 * i:3 * i + 3] = [qstep(c) for c in coefs]
+                    y, cb, cr = (p.reshape(big, big) for p in plan.download())
+                x0[i:i + 4] = quarters(y)
+                # the component images: every 2x2 group of the upsampled start image is one value
+                x0[i + 4] = cb[::2, ::2]
+                x0[i + 5] = cr[::2, ::2]
+                qs[i:i + 4] = quant_table(coefs[0])
+                qs[i + 4] = quant_table(coefs[1])
+                qs[i + 5] = quant_table(coefs[2])
             with Batch(imgs, streams=4, tune=False) as b:
                 b.run()
                 for k in range(len(imgs)):
-                    i = c0 + k
-                    target[3 * i:3 * i + 3] = np.stack(b.download(k)).reshape(3, size, size)
-            if log and ((c0 + chunk) % 2048 == 0 or c0 + chunk >= count):
+                    i = SAMPLES_PER_CROP * (c0 + k)
+                    y, cb, cr = (p.reshape(big, big).astype(np.float64) for p in b.download(k))
+                    target[i:i + 4] = quarters(y)
+                    for j, p in ((4, cb), (5, cr)):
+                        target[i + j] = 0.25 * (p[0::2, 0::2] + p[1::2, 0::2] + p[0::2, 1::2] + p[1::2, 1::2])
+            if log and ((c0 + chunk) % 1024 == 0 or c0 + chunk >= count):
                 log(f"solved {min(c0 + chunk, count)} of {count} crops", flush=True)
     return x0, qs, target
 
@@ -201,6 +236,7 @@ class WarmResult:
     objective: float
     used_fallback: bool
     warm_objective: float
+    warm_planes: list  # the refined prediction, also when the plain result is returned
 
 
 def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective: float | None = None,
@@ -223,13 +259,13 @@ def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective:
     else:
         plain = None
     if fw <= plain_objective:
-        return WarmResult(planes, fw, False, fw)
+        return WarmResult(planes, fw, False, fw, planes)
     if plain is None:
         with Plan(coefs, weight, list(pweight), plain_iterations, W, H) as plan:
             plan.init()
             plan.iterate()
             plain = plan.download()
-    return WarmResult(plain, plain_objective, True, fw)
+    return WarmResult(plain, plain_objective, True, fw, planes)
 
 
 def main(argv=None) -> int:
@@ -237,7 +273,7 @@ def main(argv=None) -> int:
     import argparse
 
     ap = argparse.ArgumentParser(description="train the warm-start CNN on solver-generated pairs")
-    ap.add_argument("--crops", type=int, default=20000)
+    ap.add_argument("--crops", type=int, default=4000, help="crops of 2 size x 2 size (six samples each)")
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--steps", type=int, default=30000)
     ap.add_argument("--batch", type=int, default=16)
@@ -260,7 +296,8 @@ def main(argv=None) -> int:
           f"(identity map: {base:.6g})", flush=True)
     net.save(args.out, crops=args.crops, size=args.size, steps=args.steps, batch=args.batch, lr=args.lr,
              target_iterations=args.target_iterations, seed=args.seed, data_seconds=t_data,
-             train_seconds=t_train, final_loss=float(np.mean(losses[-50:])), identity_loss=base)
+             train_seconds=t_train, final_loss=float(np.mean(losses[-50:])), identity_loss=base,
+             input_features=INPUT_FEATURES)
     net.close()
     return 0
 

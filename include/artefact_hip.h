@@ -217,33 +217,36 @@ int artefact_hip_process(const artefact_process_params *params, const artefact_c
 
 /* ---- Learned warm start (no reference counterpart) ----
  * A residual CNN (five 3x3 convolutions, widths 2 -> 32 -> 32 -> 32 -> 32 -> 1, ReLU, output
- * added to the decoded plane) that predicts the converged plane from the decoded one and the
- * channel's mean quantisation step.  Parameters are a flat f32 vector: per layer w[cout][cin][3][3]
- * then b[cout].  Forward, backward and Adam run as HIP kernels (BF16 MFMA, f32 accumulation). */
+ * added to the decoded plane) that predicts the converged plane from two features per pixel:
+ * the decoded plane and its quantisation-step map, the plane's 64-entry table (natural order)
+ * tiled over its 8x8 blocks, q[(y mod 8) * 8 + (x mod 8)].  Parameters are a flat f32 vector:
+ * per layer w[cout][cin][3][3] then b[cout].  Forward, backward and Adam run as HIP kernels
+ * (BF16 MFMA, f32 accumulation).  `quant` arguments hold 64 floats per plane. */
 typedef struct artefact_cnn artefact_cnn;
 uint32_t artefact_cnn_param_count(void);
 int artefact_cnn_create(const float *params, artefact_cnn **out_net);
 int artefact_cnn_get_params(artefact_cnn *net, float *out_params);
-/* Prediction (image units) for `batch` host planes of h x w with one quantisation step each,
+/* Prediction (image units) for `batch` host planes of h x w with their quantisation tables,
  * through the inference path of artefact_hip_plan_warm_init: one streaming kernel for all five
  * layers (ARTEFACT_CNN_LAYERED=1 in the environment selects one kernel per layer instead). */
-int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *qsteps, uint32_t batch, uint32_t h,
+int artefact_cnn_forward(artefact_cnn *net, const float *x0, const float *quant, uint32_t batch, uint32_t h,
                          uint32_t w, float *out_pred);
 /* Mean squared error (in units of 1/64) and its parameter gradients for a host batch; w must
  * be a multiple of 16.  grads may be NULL. */
-int artefact_cnn_gradients(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+int artefact_cnn_gradients(artefact_cnn *net, const float *x0, const float *quant, const float *target,
                            uint32_t batch, uint32_t h, uint32_t w, double *loss, float *grads);
-/* Keeps a training set (count planes of h x w, targets, steps) resident in HBM. */
-int artefact_cnn_set_dataset(artefact_cnn *net, const float *x0, const float *qsteps, const float *target,
+/* Keeps a training set (count planes of h x w, their tables, targets) resident in HBM. */
+int artefact_cnn_set_dataset(artefact_cnn *net, const float *x0, const float *quant, const float *target,
                              uint32_t count, uint32_t h, uint32_t w);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the samples `indices` of the set. */
 int artefact_cnn_train_step(artefact_cnn *net, const uint32_t *indices, uint32_t batch, float lr, double *loss);
 void artefact_cnn_destroy(artefact_cnn *net);
 
-/* Warm start of a plan: the decoded start image, the CNN's prediction for every channel
- * plane, one projection into the quantisation box (compute_projection.rs:8-122), and that
- * image as both x and x_prev.  artefact_hip_plan_iterate then runs the plan's iterations
- * from there. */
+/* Warm start of a plan: the CNN's prediction for every channel's decoded plane at its
+ * component resolution (subsampled chroma at its own size), upsampled bilinearly onto the grid
+ * (the decoded start image replicates, pipeline_scalar/coef.rs:69-92), one projection into the
+ * quantisation box (compute_projection.rs:8-122), and that image as both x and x_prev.
+ * artefact_hip_plan_iterate then runs the plan's iterations from there. */
 int artefact_hip_plan_warm_init(artefact_hip_plan *plan, artefact_cnn *net);
 /* The iteration count the constant step is derived from: step = sqrt(N)/2/sqrt(1 + n)
  * (pipeline_scalar/mod.rs:45,64-74 uses the plan's own iteration count; 0 restores that).

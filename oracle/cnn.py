@@ -14,6 +14,17 @@ import numpy as np
 WIDTHS = (2, 32, 32, 32, 32, 1)
 
 
+def features(x0, quant):
+    """Input features of planes x0 (B, H, W) with quantisation tables quant (B, 64), natural order:
+    channel 0 = x0 / 64, channel 1 = q[(y mod 8) * 8 + (x mod 8)] / 64 (the table tiled over the
+    plane's 8x8 blocks)."""
+    x0 = np.asarray(x0, np.float64)
+    B, H, W = x0.shape
+    q = np.asarray(quant, np.float64).reshape(B, 8, 8)
+    qmap = np.tile(q, (1, -(-H // 8), -(-W // 8)))[:, :H, :W]
+    return np.stack([x0 / 64.0, qmap / 64.0], axis=-1)
+
+
 def init_params(seed: int = 0, widths=WIDTHS):
     """He-initialised weights w[l]: (cout, cin, 3, 3) and zero biases b[l]: (cout,)."""
     rng = np.random.default_rng(seed)

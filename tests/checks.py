@@ -46,3 +46,14 @@ def spread(planes, ref) -> tuple[float, float]:
         within += int((d <= 0.05).sum())
         n += d.size
     return worst, within / n
+
+
+def subsample_psnr(planes, sub_file: str, W: int, H: int) -> float:
+    """PSNR (peak 255) of planes against a committed subsample of a reference result (an .npz
+    of planes p0, p1, ... holding every ``step``-th row and column from ``offset``)."""
+    with np.load(sub_file, allow_pickle=False) as z:
+        s, o = int(z["step"]), int(z["offset"])
+        ref = [z[f"p{c}"].astype(np.float64) for c in range(len(planes))]
+    mse = np.mean([np.mean((np.asarray(p, np.float64).reshape(H, W)[o::s, o::s] - r) ** 2)
+                   for p, r in zip(planes, ref)])
+    return float(10 * np.log10(255.0 ** 2 / mse))

@@ -8,10 +8,15 @@
     the f32 loop, plus the objective of the FP64 loop (``oracle.compute_f64``).
   - ``batch64``: the 64-image batch (``synth.batch_workload(64, seed=0)``) at 3 iterations:
     per image the SHA-256 of its planes (f32 loop, f64 gradient norm).
+  - ``warm_420_2048_q50``: 2048x1536 4:2:0 TGV, IJG q50, seed 0 (the learned warm start's
+    evaluation image): the FP64 loop run for 300 iterations (its converged reference), with
+    its objective.
 * ``fp64_444_4096_sub16.npz``: the FP64 loop's planes of ``tgv_444_4096``, every 16th row
   and column (offset 7), as float32 (plane ``p<c>``, shape (192, 256)).
+* ``fp64_warm_420_2048_sub8.npz``: the converged FP64 planes of ``warm_420_2048_q50``, every
+  8th row and column (offset 3), as float32 (shape (192, 256)), for PSNR estimates.
 
-    python tests/golden/make_config_fixtures.py [tgv_420_2048] [tgv_444_4096] [batch64]
+    python tests/golden/make_config_fixtures.py [tgv_420_2048] [tgv_444_4096] [batch64] [warm_420_2048_q50]
 
 With no argument every part is written; a part named on the command line is recomputed and
 the others are kept from the existing file.
@@ -83,8 +88,27 @@ def batch64():
     return rec
 
 
+WARM_SUB = os.path.join(HERE, "fp64_warm_420_2048_sub8.npz")
+
+
+def warm_converged():
+    """The warm start's evaluation image: FP64 loop, 300 iterations (its converged reference)."""
+    W, H, it = 2048, 1536, 300
+    coefs = synth.encode(synth.synth_rgb(H, W, 0), subsampling="420", quality=50)
+    t0 = time.time()
+    out = oracle.compute_f64(coefs, 0.3, [0.001] * 3, it, W, H)
+    f, terms = oracle.objective(coefs, 0.3, [0.001] * 3, W, H, out)
+    sub = {f"p{c}": p.reshape(H, W)[3::8, 3::8].astype(np.float32) for c, p in enumerate(out)}
+    np.savez_compressed(WARM_SUB, step=8, offset=3, **sub)
+    rec = {"seed": 0, "width": W, "height": H, "quality": 50, "iterations": it, "fp64_objective": f,
+           "fp64_terms": [float(t) for t in terms], "fp64_seconds": round(time.time() - t0, 1),
+           "fp64_subsample": {"file": os.path.basename(WARM_SUB), "step": 8, "offset": 3}}
+    print("warm_420_2048_q50", rec["fp64_seconds"], "s", flush=True)
+    return rec
+
+
 def main(argv):
-    parts = argv or ["tgv_420_2048", "tgv_444_4096", "batch64"]
+    parts = argv or ["tgv_420_2048", "tgv_444_4096", "batch64", "warm_420_2048_q50"]
     res = {}
     if os.path.exists(OUT):
         with open(OUT) as f:
@@ -96,6 +120,8 @@ def main(argv):
             res[p] = single(p, fp64=True)
         elif p == "batch64":
             res[p] = batch64()
+        elif p == "warm_420_2048_q50":
+            res[p] = warm_converged()
         else:
             raise SystemExit(f"unknown part {p}")
         with open(OUT, "w") as f:

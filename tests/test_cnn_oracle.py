@@ -61,3 +61,18 @@ def test_adam_first_step_moves_by_lr():
     m, v = np.zeros(2), np.zeros(2)
     cnn.adam_step(p, g, m, v, 1, 0.01)
     assert np.allclose(p, [1.0 - 0.01, -2.0 + 0.01], atol=1e-7)  # |mhat/sqrt(vhat)| = 1 at t = 1
+
+
+def test_features_tile_the_table_over_the_blocks():
+    """Feature 1 at pixel (y, x) is the plane's table entry (y mod 8, x mod 8) / 64; feature 0 is
+    the plane / 64."""
+    rng = np.random.default_rng(4)
+    x0 = rng.normal(size=(2, 13, 21))
+    q = rng.uniform(1, 99, size=(2, 64))
+    f = cnn.features(x0, q)
+    assert f.shape == (2, 13, 21, 2)
+    assert np.array_equal(f[..., 0], x0 / 64.0)
+    for b in range(2):
+        for y in range(13):
+            for x in range(21):
+                assert f[b, y, x, 1] == q[b, (y % 8) * 8 + x % 8] / 64.0

@@ -15,12 +15,7 @@ from oracle import cnn
 pytestmark = pytest.mark.gpu
 
 
-def features(x0, q):
-    B, H, W = x0.shape
-    f = np.zeros((B, H, W, 2))
-    f[..., 0] = x0 / 64.0
-    f[..., 1] = (np.asarray(q)[:, None, None] / 64.0) * np.ones((B, H, W))
-    return f
+features = cnn.features
 
 
 def params_with_bias(seed):
@@ -37,7 +32,7 @@ def batch(seed, B=2, H=24, W=48):
     rng = np.random.default_rng(seed)
     yy, xx = np.mgrid[0:H, 0:W]
     x0 = np.stack([60 * np.sin(xx / (5 + i)) + 40 * np.cos(yy / 7) + rng.normal(0, 8, (H, W)) for i in range(B)])
-    q = rng.uniform(5, 40, size=B)
+    q = rng.uniform(2, 80, size=(B, 64))  # a quantisation table per plane
     t = x0 + rng.normal(0, 4, x0.shape)
     return x0.astype(np.float32), q.astype(np.float32), t.astype(np.float32)
 
@@ -128,3 +123,26 @@ def test_streaming_kernel_matches_layered(H, W, monkeypatch):
     print(f"relative L2 {rel:.3g}, max {mx:.3g} of the range")
     assert rel <= 0.01
     assert mx <= 0.02
+
+
+def test_failed_workspace_allocation_leaves_the_net_usable():
+    """A forward too large to allocate returns ARTEFACT_E_HIP, and the net's workspace pointers are
+    cleared as they are released: a normal forward afterwards succeeds and destroy frees nothing
+    twice.  The input buffers are never read: the allocation fails first."""
+    import ctypes
+
+    from artefact_amd import _lib
+
+    flat, _, _ = params_with_bias(2)
+    net = warmstart.WarmStartNet(flat)
+    x0, q, _ = batch(6, B=1, H=24, W=32)
+    net.forward(x0, q)  # allocates a small workspace
+    lib = _lib.load()
+    F = ctypes.POINTER(ctypes.c_float)
+    tiny = np.zeros(64 * 64, np.float32)
+    rc = lib.artefact_cnn_forward(net._h, tiny.ctypes.data_as(F), tiny.ctypes.data_as(F), 64, 65535, 65535,
+                                  tiny.ctypes.data_as(F))
+    assert rc == -4
+    again = net.forward(x0, q)
+    assert np.isfinite(again).all()
+    net.close()

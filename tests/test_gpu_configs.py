@@ -29,7 +29,7 @@ import pytest
 import artefact_amd
 import oracle
 from artefact_amd import synth, warmstart
-from checks import bits_equal, digest, requant_deviation, spread
+from checks import bits_equal, digest, requant_deviation, spread, subsample_psnr
 
 pytestmark = pytest.mark.gpu
 
@@ -189,3 +189,16 @@ def test_warm_start_full_size():
         for a, b in zip(res.planes, plain):
             assert bits_equal(a, b)
     assert requant_deviation(res.planes, coefs, W, H) <= 0.5 + 1e-3
+    assert requant_deviation(res.warm_planes, coefs, W, H) <= 0.5 + 1e-3
+    # Distance to the converged result (the FP64 loop after 300 iterations, committed as a
+    # subsample): the refined prediction is well past the decode, plain 100 further still.
+    fx = fixture("warm_420_2048_q50")["fp64_subsample"]
+    sub = os.path.join(GOLDEN, fx["file"])
+    with artefact_amd.Plan(coefs, 0.3, [0.001] * 3, 1, W, H) as p:
+        p.init()
+        decoded = p.download()
+    p_dec, p_warm, p_plain = (subsample_psnr(p, sub, W, H) for p in (decoded, res.warm_planes, plain))
+    print(f"PSNR against the converged FP64 loop: decoded {p_dec:.2f} dB, CNN + 15 {p_warm:.2f} dB, "
+          f"plain 100 {p_plain:.2f} dB", flush=True)
+    assert p_warm > p_dec + 3.0
+    assert p_plain > p_warm
