@@ -221,6 +221,256 @@ def cpu_baseline(wl, iters, runs=3):
     }
 
 
+def measure_plan(wl, precision="f32", steps=5, warmup=2):
+    """One workload solved steps times from resident coefficients (init + graph of the loop) after
+    autotuning: per-solve and per-iteration time (HIP events around the graph launches), the
+    roofline fraction of the iteration, the objective and the planes."""
+    import artefact_amd
+
+    W, H, iters = wl["max_w"], wl["max_h"], wl["iterations"]
+    plan = artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"], graph=True)
+    plan.set_precision(precision)
+    tuning = plan.tune()
+    for _ in range(warmup):
+        plan.init()
+        plan.iterate()
+    plan.download()
+    plan.set_timing(steps)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.init()
+        plan.iterate()
+    plan.last_norms()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    loop_ms, launches = plan.timing()
+    plan.set_timing(0)
+    plan.init()
+    plan.iterate()
+    objective, _ = plan.objective()
+    out = plan.download()
+    plan.close()
+    t_iter = loop_ms / (launches * iters)
+    B = algorithmic_bytes(wl["coefs"], W, H)
+    achieved = B / (t_iter * 1e-3) / 1e9
+    return {"ms_per_solve": round(ms, 4), "iteration_ms": round(t_iter, 5),
+            "Mpixel_it_s": round(W * H * iters / (ms * 1e-3) / 1e6, 2),
+            "roofline": {"achieved_GBps": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_iteration": B},
+            "objective": objective, "autotuned": tuning}, out
+
+
+def measure_batch(imgs, steps=5, warmup=2, tune=True):
+    """The images in one graph (4 streams) against the same images solved one after another,
+    each as its own graph-captured plan."""
+    import artefact_amd
+
+    work = sum(w["max_w"] * w["max_h"] * w["iterations"] for w in imgs)
+    B = sum(algorithmic_bytes(w["coefs"], w["max_w"], w["max_h"]) * w["iterations"] for w in imgs)
+    images = [artefact_amd.BatchImage(w["coefs"], w["weight"], w["pweight"], w["iterations"], w["max_w"], w["max_h"],
+                                      w["mode"]) for w in imgs]
+    t0 = time.perf_counter()
+    batch = artefact_amd.Batch(images, streams=4, tune=tune)
+    setup_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        batch.run()
+    batch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run()
+    batch.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    batch.close()
+    plans = [artefact_amd.Plan(w["coefs"], w["weight"], w["pweight"], w["iterations"], w["max_w"], w["max_h"],
+                               mode=w["mode"]) for w in imgs]
+    for pl in plans:
+        pl.init()
+        pl.iterate()
+    for pl in plans:
+        pl.last_norms()
+    reps = max(1, steps // 2)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for pl in plans:
+            pl.init()
+            pl.iterate()
+            pl.last_norms()
+    seq_ms = 1e3 * (time.perf_counter() - t0) / reps
+    for pl in plans:
+        pl.close()
+    achieved = B / (ms * 1e-3) / 1e9
+    return {"images": len(imgs), "pixel_iterations": work, "ms_per_batch": round(ms, 3),
+            "Mpixel_it_s": round(work / (ms * 1e-3) / 1e6, 2), "sequential_ms": round(seq_ms, 3),
+            "batch_speedup_vs_sequential": round(seq_ms / ms, 3), "setup_s": round(setup_s, 2),
+            "roofline": {"achieved_GBps": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes_per_batch": B}}
+
+
+def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
+    """CNN prediction -> projection -> K plain iterations (the step of the 100-iteration schedule)
+    against the plain 100-iteration solve: times, objectives, the fallback decision, and the
+    planes (decoded, plain 100, warm, plain K)."""
+    import artefact_amd
+
+    pw = [0.001] * 3
+
+    def timed(plan, start):
+        for _ in range(warmup):
+            start(plan)
+            plan.iterate()
+        plan.last_norms()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            start(plan)
+            plan.iterate()
+        plan.last_norms()
+        return 1e3 * (time.perf_counter() - t0) / steps
+
+    def warm_start(p):
+        artefact_amd._lib.check(p._lib.artefact_hip_plan_warm_init(p._h, net._h), "artefact_hip_plan_warm_init")
+
+    def cnn_only(p):  # the prediction and its projection, no iteration
+        warm_start(p)
+
+    res, planes = {}, {}
+    plain = artefact_amd.Plan(coefs, 0.3, pw, 100, W, H)
+    res["plain_100_ms"] = round(timed(plain, lambda p: p.init()), 4)
+    plain.init()
+    plain.iterate()
+    res["plain_100_objective"], _ = plain.objective()
+    planes["plain_100"] = plain.download()
+    plain.init()
+    planes["decoded"] = plain.download()
+    plain.close()
+    warm = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
+    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, 100), "step")
+    res[f"cnn_plus_{K}_ms"] = round(timed(warm, warm_start), 4)
+    warm_start(warm)
+    warm.iterate()
+    res[f"cnn_plus_{K}_objective"], _ = warm.objective()
+    planes[f"cnn_plus_{K}"] = warm.download()
+    warm.close()
+    zero = artefact_amd.Plan(coefs, 0.3, pw, 0, W, H)
+    for _ in range(warmup):
+        cnn_only(zero)
+    zero.last_norms()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cnn_only(zero)
+    zero.last_norms()
+    res["cnn_and_projection_ms"] = round(1e3 * (time.perf_counter() - t0) / steps, 4)
+    zero.close()
+    cold = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
+    res[f"plain_{K}_ms"] = round(timed(cold, lambda p: p.init()), 4)
+    cold.init()
+    cold.iterate()
+    res[f"plain_{K}_objective"], _ = cold.objective()
+    planes[f"plain_{K}"] = cold.download()
+    cold.close()
+    f_plain, f_warm = res["plain_100_objective"], res[f"cnn_plus_{K}_objective"]
+    res["warm_time_fraction_of_plain"] = round(res[f"cnn_plus_{K}_ms"] / res["plain_100_ms"], 4)
+    res["warm_objective_rel_to_plain"] = float(f"{(f_warm - f_plain) / f_plain:.4g}")
+    res["reaches_plain_objective_within_0.1pct"] = bool(f_warm <= f_plain * 1.001)
+    res["used_fallback"] = bool(f_warm > f_plain)
+    return res, planes
+
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def golden(name):
+    """The oracle's committed results for a full-size workload (tests/golden/make_config_fixtures.py)."""
+    path = os.path.join(GOLDEN, "config_results.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(name)
+
+
+def subsample_psnr(planes, fx, W, H):
+    """PSNR (0..255 scale) of planes against a committed FP64 subsample."""
+    import numpy as np
+
+    sub = fx["fp64_subsample"]
+    with np.load(os.path.join(GOLDEN, sub["file"]), allow_pickle=False) as z:
+        ref = [z[f"p{c}"].astype(np.float64) for c in range(len(planes))]
+    s, o = sub["step"], sub["offset"]
+    mse = np.mean([np.mean((p.reshape(H, W)[o::s, o::s].astype(np.float64) - r) ** 2) for p, r in zip(planes, ref)])
+    return round(float(10 * np.log10(255.0 ** 2 / mse)), 3)
+
+
+def prepare_configs(args):
+    """The other configurations' inputs, synthesised before any GPU work (the batch's 64 images by a
+    pool of spawned processes)."""
+    from artefact_amd import synth
+
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    pre = {
+        "tv_luma_512": synth.workload("tv_luma_512", seed=0),
+        "tgv_444_4096": synth.workload("tgv_444_4096", seed=0),
+        "batch64": synth.batch_workload(64, seed=0, workers=workers),
+        "warm": synth.encode(synth.synth_rgb(1536, 2048, 0), subsampling="420", quality=50),
+    }
+    print(f"bench: other configurations synthesised in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    return pre
+
+
+def other_configs(args, pre):
+    """The other BASELINE configurations, timed on this GPU in the same run as the headline (a few
+    solves each): plain TV luma 512x512; the precision form at 4096x3072 4:4:4 q30 (f32 and
+    BF16x3); the 64-image batch; the learned warm start at 2048x1536 4:2:0 q50."""
+    from artefact_amd import warmstart
+
+    steps, warmup = max(2, min(args.steps, 10)), max(1, min(args.warmup, 2))
+    out = {}
+    wl = pre["tv_luma_512"]
+    r, _ = measure_plan(wl, "f32", steps, warmup)
+    out["tv_luma_512"] = {"workload": describe(wl), **r}
+    print(f"bench: tv_luma_512 {r['iteration_ms'] * 1e3:.2f} us per iteration", file=sys.stderr, flush=True)
+
+    wl = pre["tgv_444_4096"]
+    fx = golden("tgv_444_4096")
+    prec = {}
+    for p in ("f32", "bf16x3"):
+        r, planes = measure_plan(wl, p, max(2, steps // 2), warmup)
+        if fx:
+            r["objective_rel_to_fp64_oracle"] = float(f"{(r['objective'] - fx['fp64_objective']) / fx['fp64_objective']:.4g}")
+            r["psnr_vs_fp64_oracle_subsample_db"] = subsample_psnr(planes, fx, wl["max_w"], wl["max_h"])
+        prec[p] = r
+        print(f"bench: tgv_444_4096 {p} {r['ms_per_solve']:.2f} ms per solve", file=sys.stderr, flush=True)
+    out["precision_444_4096"] = {"workload": describe(wl) + "; bf16x3: projection DCT/IDCTs on BF16 MFMA, hi/lo "
+                                 "split, f32 accumulation", "by_precision": prec,
+                                 "fp64_reference": "tests/golden/config_results.json (oracle.compute_f64, 100 "
+                                                   "iterations; subsample every 16th row and column)" if fx else None}
+
+    b = measure_batch(pre["batch64"], max(2, steps // 2), warmup)
+    out["batch64"] = {"workload": "64 images, sides uniform in 256..2048 (multiples of 16), half 4:2:0 / half 4:4:4, "
+                                  "IJG q30..90, half TV / half TGV, 100 iterations, one hipGraph over 4 streams, "
+                                  "autotuned per geometry, seed 0", **b}
+    print(f"bench: batch64 {b['ms_per_batch']:.1f} ms per batch", file=sys.stderr, flush=True)
+
+    try:
+        net = warmstart.WarmStartNet.load()
+    except (OSError, ValueError) as e:
+        out["warm_start_420_2048_q50"] = {"error": str(e)}
+        return out
+    coefs = pre["warm"]
+    W, H = coefs[0].rounded_px_w, coefs[0].rounded_px_h
+    w, planes = measure_warm(net, coefs, W, H, 15, steps, warmup)
+    net.close()
+    fx = golden("warm_420_2048_q50")
+    if fx:
+        w["psnr_vs_fp64_converged_subsample_db"] = {k: subsample_psnr(v, fx, W, H) for k, v in planes.items()}
+        w["fp64_converged_reference"] = ("tests/golden/config_results.json (oracle.compute_f64, 300 iterations; "
+                                         "subsample every 8th row and column)")
+    out["warm_start_420_2048_q50"] = {"workload": "learned warm start: TGV 4:2:0 2048x1536 IJG q50, CNN + 15 plain "
+                                                  "iterations (step of the 100-iteration schedule) vs plain 100", **w}
+    print(f"bench: warm start {w['cnn_plus_15_ms']:.3f} ms vs plain {w['plain_100_ms']:.3f} ms", file=sys.stderr,
+          flush=True)
+    return out
+
+
 def bench_precision(args) -> int:
     """Precision form on the 4:4:4 4096x3072 q30 TGV workload: the projection's transforms on
     BF16 MFMA (hi/lo, f32 accumulation); error of the result against the FP64 oracle loop per
@@ -629,6 +879,9 @@ def main() -> int:
     ap.add_argument("--precision", default="f32", choices=["f32", "f32_fast"],
                     help="arithmetic of the timed loop (default: the bit-exact f32 drop-in)")
     ap.add_argument("--no-fast", action="store_true", help="skip the f32_fast comparison of the default line")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the other configurations (512x512 TV, precision form, batch, warm start) of the "
+                         "default line")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print per-kernel HBM bytes from two rocprofv3 --pmc CSVs and exit")
     args = ap.parse_args()
@@ -654,6 +907,10 @@ def main() -> int:
     if args.workload == "process":
         return bench_process(args)
 
+    # Inputs of the other configurations first: the batch is synthesised by spawned processes,
+    # which must start before this process touches the GPU.
+    extras = args.workload == "tgv_420_2048" and not args.no_extras
+    pre = prepare_configs(args) if extras else None
     wl = synth.workload(args.workload, seed=0)
     W, H, iters = wl["max_w"], wl["max_h"], wl["iterations"]
     coefs = wl["coefs"]
@@ -754,6 +1011,8 @@ def main() -> int:
         res["dtype"] = f"f32 ({args.precision}: hardware sqrt/reciprocal in the gradient sweep, not bit-exact)"
     if fast is not None:
         res["fast_f32"] = fast
+    if extras:
+        res["configs"] = other_configs(args, pre)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, args.cpu_iters)
     print(json.dumps(res), flush=True)
