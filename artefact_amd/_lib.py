@@ -86,7 +86,7 @@ class artefact_tuning(ctypes.Structure):
         ("kg_rows", ctypes.c_uint32),
         ("project_kernel", ctypes.c_uint32),
         ("ms_per_iteration", ctypes.c_float),
-        ("fused_norm", ctypes.c_uint32),
+        ("project_workgroups", ctypes.c_uint32),
     ]
 
 

@@ -1,17 +1,17 @@
 // artefact_hip: the reconstruction loop of artefact-lib as hand-written gfx950 kernels.
 //
-// One iteration of backend/artefact-lib/pipeline_scalar/mod.rs:49-75 is three kernels:
+// One iteration of backend/artefact-lib/pipeline_scalar/mod.rs:49-75 is two kernels:
 //
 //   k_gradient  FISTA extrapolation (mod.rs:54-59) + TV gradient (compute_step_tv.rs) +
 //               second-order gradient (compute_step_tv2.rs) + the prob gradient
 //               (compute_step_prob.rs, read from the plane k_project left behind).
 //               A register-streaming stencil: each lane owns one image column and walks
 //               down a strip of rows; horizontal neighbours come from DPP lane shifts,
-//               so there is no LDS staging and no barrier in the stencil.  Per-workgroup
-//               sums of g^2 in double-double go to a partials array.
-//   k_finalize  One workgroup reduces the partials, in a fixed order, to the per-channel
-//               gradient norm (compute_step.rs:54-58).
-//   k_project   FISTA extrapolation again (recomputed, never stored), the normalised
+//               so there is no LDS staging and no barrier in the stencil.  Each
+//               workgroup's sum of g^2 per channel (double-double) is added, exactly, into
+//               a fixed-point integer accumulator with 64-bit atomics.
+//   k_project   Every workgroup first forms the gradient norm (compute_step.rs:54-58) from
+//               the accumulator, then the normalised
 //               descent step (compute_step.rs:61-65) and the box projection
 //               (compute_projection.rs): chroma box mean, 8x8 DCT, clamp to the
 //               quantisation interval, 8x8 IDCT, mean re-add.  It also evaluates the next
@@ -23,7 +23,8 @@
 // -ffp-contract=off), so their output equals the CPU restatement bit for bit once the
 // gradient norm is computed the same way.  The gradient norm is the one place where the
 // order differs: the reference folds N squares sequentially in f32 (compute_step.rs:54-58);
-// here the squares are summed in double-double and rounded once.
+// here the squares are summed in f64 per lane, merged in double-double per workgroup, and the
+// workgroups' sums are added exactly (fixed point) and rounded once.
 //
 // Gather form of the scatter loops.  The reference scatters from every source pixel p
 // into up to 3 (TV) and 7 (second-order) neighbours, in raster order of p.  Every
@@ -263,14 +264,91 @@ struct IterArgs {
     float *g[3];
     float *xnew[3];
     ChanDev ch[3];
-    double *partials;        // [workgroups][3][2]: double-double sums of g^2
+    long long *acc;          // ACC_WORDS: this iteration's norm accumulator (k_gradient adds)
+    long long *acc_next;     // ACC_WORDS: the next iteration's, cleared by the projection
     float *norms;            // [3]
     double *rnorm;           // [3] 1/norm, correctly rounded f64
     double *norm_sq;         // [3]
     uint32_t kg_rows;        // k_gradient output rows per wave
-    uint32_t fuse_norm;      // 1: the projection reduces the norms itself (no k_finalize launch)
-    uint32_t nwg_grad;       // k_gradient workgroups (partials)
 };
+
+// ---- Norm accumulator ----
+// The sum of g^2 per channel over all k_gradient workgroups, kept exactly as a fixed-point
+// integer whose unit is 2^-352: ACC_DIGITS digits of 32 bits, digit d of weight 2^(32 d - 352),
+// each held in an int64 that takes signed 32-bit chunks by atomic addition (carries are
+// resolved by the reader), plus a flag word counting values that do not fit.  Every value a
+// workgroup adds is a multiple of 2^-298 (g is an f32, a multiple of 2^-149, so g^2 and every
+// rounded sum or double-double error term of such squares is too) and below 2^288 (|g| <
+// 2^128 and at most 2^29 pixels), so it fits the 640 bits exactly.  Integer addition is
+// associative: the total does not depend on the order in which the workgroups finish.
+// ACC_SLOTS copies spread the workgroups' atomics over more addresses.
+constexpr int ACC_DIGITS = 20;
+constexpr int ACC_CH = ACC_DIGITS + 1;  // digits, then the flag
+constexpr int ACC_SLOTS = 8;
+constexpr int ACC_WORDS = ACC_SLOTS * 3 * ACC_CH;  // one accumulator (4032 bytes)
+
+// Adds chunk j (0..2) of the finite double x to the accumulator of one channel (ACC_CH words).
+__device__ __forceinline__ void acc_add(long long *acc, double x, int j)
+{
+    if (x == 0.0) return;
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+    const int ex = (int)((bits >> 52) & 0x7ff);
+    // The value range above: 2^-298 <= |x| < 2^288 (biased exponents 725 .. 1310).  Anything
+    // else (a non-finite x included) is counted in the flag, which makes the norm NaN.
+    if (ex < 723 || ex > 1310) {
+        if (j == 0) __hip_atomic_fetch_add(acc + ACC_DIGITS, 1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    const unsigned long long m = (bits & ((1ull << 52) - 1)) | (1ull << 52);
+    const int pos = ex - 1075 + 352;  // bit position of m's lowest bit, >= 0
+    const int q = pos >> 5, sh = pos & 31;
+    const unsigned long long lo = m << sh, hi = sh ? m >> (64 - sh) : 0ull;
+    const unsigned long long chunk = j == 0 ? (lo & 0xffffffffull) : (j == 1 ? lo >> 32 : hi);
+    const int d = q + j;
+    if (chunk == 0 || d >= ACC_DIGITS) return;  // a chunk above the top digit is 0 (|x| < 2^288)
+    const long long v = (long long)chunk;
+    __hip_atomic_fetch_add(acc + d, (long long)(bits >> 63) ? -v : v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The accumulated sum of one channel (the ACC_SLOTS copies already added, in `dig`), rounded
+// once to f64 (to nearest, ties to even); NaN if a value was out of range.
+__device__ __forceinline__ double acc_value(const long long *dig)
+{
+    if (dig[ACC_DIGITS] != 0) return __builtin_nan("");
+    unsigned int d32[ACC_DIGITS];
+    long long carry = 0;
+#pragma unroll
+    for (int i = 0; i < ACC_DIGITS; i++) {
+        const long long v = dig[i] + carry;
+        d32[i] = (unsigned int)(v & 0xffffffffll);
+        carry = v >> 32;  // arithmetic shift: a negative running total borrows
+    }
+    if (carry != 0) return __builtin_nan("");  // a negative total (impossible for sums of squares)
+    int k = -1;
+#pragma unroll
+    for (int i = 0; i < ACC_DIGITS; i++)
+        if (d32[i]) k = i;
+    if (k < 0) return 0.0;
+    // The top three digits as a 96-bit integer T:R, the digits below as a sticky bit.
+    unsigned long long T = ((unsigned long long)d32[k] << 32) | (k >= 1 ? d32[k - 1] : 0u);
+    unsigned int R = k >= 2 ? d32[k - 2] : 0u;
+    bool sticky = false;
+#pragma unroll
+    for (int i = 0; i < ACC_DIGITS; i++)
+        if (i < k - 2 && d32[i]) sticky = true;
+    const int lz = __builtin_clzll(T);  // < 32: the top digit is non-zero
+    if (lz) {
+        T = (T << lz) | ((unsigned long long)R >> (32 - lz));
+        R <<= lz;
+    }
+    unsigned long long mant = T >> 11;
+    const unsigned int rest = (unsigned int)(T & 0x7ff);
+    const bool round = (rest >> 10) & 1u;
+    const bool below = (rest & 0x3ffu) != 0 || R != 0 || sticky;
+    if (round && (below || (mant & 1ull))) mant++;
+    // value = mant * 2^(32 (k - 2) + 32 + 11 - lz) in units of 2^-352
+    return ldexp((double)mant, 32 * (k - 2) + 43 - lz - 352);
+}
 
 // Raw buffer resource over `bytes` bytes: loads outside return 0, stores outside are dropped.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *p, uint32_t bytes)
@@ -628,9 +706,9 @@ __global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
     if (k < kend) step(k, IC<1>{}, IC<4>{});
     if (k + 1 < kend) step(k + 1, IC<2>{}, IC<4>{});
 
-    // Per-workgroup partial sums, in a fixed order: wave c merges channel c across the
-    // workgroup's waves lane by lane, then across its lanes; k_finalize reduces the
-    // workgroup partials.
+    // Per-workgroup sums: wave c merges channel c across the workgroup's waves lane by lane,
+    // then across its lanes, and adds lane 0's double-double result to the accumulator (lanes
+    // 0-2 the chunks of its high part, lanes 3-5 those of its low part).
 #pragma unroll
     for (int c = 0; c < C; c++) lred[wave][c][lane] = acc[c];
     __syncthreads();
@@ -641,130 +719,79 @@ __global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
         for (int w = 1; w < WAVES; w++) v = dd_merge(v, lred[w][c][lane]);
 #pragma unroll
         for (int m = 32; m >= 1; m >>= 1) v = dd_merge(v, dd_shfl_xor(v, m));
-        if (lane == 0) {
+        const double hi = __shfl(v.hi, 0, 64), lo = __shfl(v.lo, 0, 64);
+        if (lane < 6) {
             const unsigned int wg = ty * gridDim.x + tx;
-            a.partials[(size_t)wg * 6 + c * 2 + 0] = v.hi;
-            a.partials[(size_t)wg * 6 + c * 2 + 1] = v.lo;
+            long long *acc = a.acc + (size_t)(wg % ACC_SLOTS) * 3 * ACC_CH + (size_t)c * ACC_CH;
+            acc_add(acc, lane < 3 ? hi : lo, lane < 3 ? lane : lane - 3);
         }
     }
 }
 
-// Gradient norm per channel from the workgroup partials (compute_step.rs:54-58) in a fixed
-// reduction order, so the result is the same on every run and in every workgroup that forms
-// it.  Called by all NF threads of a workgroup; on return (after a barrier) out.norm[c] and
-// out.rnorm[c] hold the results in LDS, and if `writer` is set the workgroup has also stored
-// them, with the squared norm, to the plan's buffers.
-constexpr int NF = 256;
+// Gradient norm per channel (compute_step.rs:54-58) from the accumulator k_gradient filled,
+// so every workgroup that forms it gets the same bits, whatever order k_gradient's workgroups
+// ran in.  Called by all threads of a workgroup (it has barriers); on return out.norm[c] and
+// out.rnorm[c] hold the results in LDS.  If `writer` is set, the workgroup also stores them,
+// with the squared norm, to the plan's buffers, and clears the next iteration's accumulator
+// (which no kernel of this iteration touches).  k_gradient adds with device-scope atomics;
+// the reads and the clearing are plain accesses in the kernels before and after it, ordered
+// by the kernel boundaries like every other buffer the kernels hand on.
 struct NormShared {
-    DD red[NF / 64][3];
+    long long dig[3][ACC_CH];
     float norm[3];
     double rnorm[3];
 };
-// BATCH partials per thread are loaded before they are merged (k_finalize: 8, to have many
-// loads in flight; a projection kernel: 4, to keep its register budget).  The result is the
-// same for every BATCH: thread t merges partials t, t + NF, t + 2 NF, ... in that order
-// (a zero pair past the last one changes nothing).
-// PLAIN (the fast f32 mode, whose partials carry no low part): the same tree with plain f64
-// additions, a much shorter dependent chain.
-template <int C, int BATCH, bool PLAIN = false>
-__device__ __forceinline__ void reduce_norms(const double *partials, unsigned int nwg, bool writer, float *norms,
-                                             double *rnorm, double *norm_sq, NormShared &out)
+template <int C>
+__device__ __forceinline__ void accumulated_norms(const IterArgs &a, bool writer, NormShared &out)
 {
-    static_assert(8 % BATCH == 0, "BATCH divides 8");
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    DD s[C];
+    const int tid = threadIdx.x;
+    if (tid < C * ACC_CH) {
+        const int c = tid / ACC_CH, d = tid % ACC_CH;
+        long long v = 0;
 #pragma unroll
-    for (int c = 0; c < C; c++) s[c] = DD{0.0, 0.0};
-#pragma unroll 1
-    for (unsigned int w0 = 0; w0 < nwg; w0 += 8 * NF) {
-#pragma unroll 1
-        for (int u0 = 0; u0 < 8; u0 += BATCH) {
-            // A batch past the last partial would merge only zero pairs, and merging a zero pair
-            // into a normalised double-double returns it unchanged, so it is skipped.
-            if (w0 + (unsigned int)u0 * NF >= nwg) break;
-            double v[BATCH][2 * C];
-#pragma unroll
-            for (int u = 0; u < BATCH; u++) {
-                const unsigned int w = w0 + (u0 + u) * NF + tid;
-#pragma unroll
-                for (int k = 0; k < 2 * C; k++) v[u][k] = w < nwg ? partials[(size_t)w * 6 + k] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < BATCH; u++)
-#pragma unroll
-                for (int c = 0; c < C; c++) {
-                    if constexpr (PLAIN) s[c].hi += v[u][2 * c];
-                    else s[c] = dd_merge(s[c], DD{v[u][2 * c], v[u][2 * c + 1]});
-                }
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < C; c++) {
-        DD v = s[c];
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-            if constexpr (PLAIN) v.hi += __shfl_xor(v.hi, m, 64);
-            else v = dd_merge(v, dd_shfl_xor(v, m));
-        }
-        if (lane == 0) out.red[wave][c] = v;
+        for (int sl = 0; sl < ACC_SLOTS; sl++) v += a.acc[(size_t)sl * 3 * ACC_CH + c * ACC_CH + d];
+        out.dig[c][d] = v;
     }
     __syncthreads();
     if (tid < C) {
         const int c = tid;
-        DD v = out.red[0][c];
-#pragma unroll
-        for (int w = 1; w < NF / 64; w++) {
-            if constexpr (PLAIN) v.hi += out.red[w][c].hi;
-            else v = dd_merge(v, out.red[w][c]);
-        }
-        const double sum = v.hi + v.lo;
+        const double sum = acc_value(out.dig[c]);
         const float norm = (float)sqrt(sum);
         const double rn = norm != 0.0f ? 1.0 / (double)norm : 0.0;
         out.norm[c] = norm;
         out.rnorm[c] = rn;
         if (writer) {
-            norm_sq[c] = sum;
-            norms[c] = norm;
-            rnorm[c] = rn;
+            a.norm_sq[c] = sum;
+            a.norms[c] = norm;
+            a.rnorm[c] = rn;
         }
     }
+    if (writer)
+        for (int i = tid; i < ACC_WORDS; i += blockDim.x) a.acc_next[i] = 0;
     __syncthreads();
 }
 
-// The reduction as a kernel of its own (one workgroup), between k_gradient and the projection.
-template <int C, bool PLAIN = false>
-__global__ __launch_bounds__(NF) void k_finalize(const double *partials, unsigned int nwg, float *norms,
-                                                 double *rnorm, double *norm_sq)
-{
-    __shared__ NormShared ns;
-    reduce_norms<C, 8, PLAIN>(partials, nwg, true, norms, rnorm, norm_sq, ns);
-}
-
-// A projection kernel's view of the norms: reduced by its own workgroup from the k_gradient
-// partials when the plan runs without k_finalize (a.fuse_norm), else read from the plan.
-// Every thread of the workgroup calls it (the fused reduction has barriers).  FUSE is a
-// template parameter so that the kernels of the separate-reduction plans carry none of it.
-// BATCH: partials loaded per thread at a time (1 keeps the MFMA kernel's register budget; the
-// lane kernel takes 2 at no register cost).
-template <int C, bool FUSE, int BATCH = 1>
+// The norms a projection kernel descends with, formed by each of its workgroups; the first
+// workgroup is the writer.
+template <int C>
 __device__ __forceinline__ void projection_norms(const IterArgs &a, NormShared &ns)
 {
-    static_assert(NT == NF, "projection workgroups are NF threads");
-    if constexpr (FUSE)
-        reduce_norms<C, BATCH>(a.partials, a.nwg_grad, (blockIdx.x | blockIdx.y) == 0, a.norms, a.rnorm, a.norm_sq,
-                               ns);
+    accumulated_norms<C>(a, (blockIdx.x | blockIdx.y) == 0, ns);
 }
-template <bool FUSE>
-__device__ __forceinline__ float proj_norm(const IterArgs &a, const NormShared &ns, int c)
+
+// Clears both accumulators of a plan (2 * ACC_WORDS words): before a loop starts, so that the
+// first iteration's sum starts from zero whatever ran before.
+__global__ __launch_bounds__(NT) void k_zero_acc(long long *acc)
 {
-    if constexpr (FUSE) return ns.norm[c];
-    else return a.norms[c];
+    for (int i = threadIdx.x; i < 2 * ACC_WORDS; i += NT) acc[i] = 0;
 }
-template <bool FUSE>
-__device__ __forceinline__ double proj_rnorm(const IterArgs &a, const NormShared &ns, int c)
+
+// The norms alone (one workgroup), for the calls that run k_gradient without a projection.
+template <int C>
+__global__ __launch_bounds__(NT) void k_norms(const IterArgs a)
 {
-    if constexpr (FUSE) return ns.rnorm[c];
-    else return a.rnorm[c];
+    __shared__ NormShared ns;
+    accumulated_norms<C>(a, true, ns);
 }
 
 struct ProjShared {
@@ -905,7 +932,7 @@ __device__ __forceinline__ void project_channel(const IterArgs &a, const ChanDev
     __syncthreads();
 }
 
-template <int C, bool FUSE>
+template <int C>
 __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
 {
     __shared__ ProjShared sh;
@@ -926,12 +953,12 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
         yv[c] = *reinterpret_cast<const float4 *>(a.y[c] + p);
         gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
     }
-    projection_norms<C, FUSE>(a, ns);
+    projection_norms<C>(a, ns);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const ChanDev &ch = a.ch[c];
-        const float norm = proj_norm<FUSE>(a, ns, c);
-        const double rn = proj_rnorm<FUSE>(a, ns, c);
+        const float norm = ns.norm[c];
+        const double rn = ns.rnorm[c];
         // Descent step on the extrapolated image (compute_step.rs:61-65).
         const float ys[4] = {yv[c].x, yv[c].y, yv[c].z, yv[c].w};
         const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
@@ -1064,7 +1091,7 @@ __device__ __forceinline__ int fused_block(int b, int x0, int y0, int &bx, int &
 
 // k_project for the common layouts (channel modes known at compile time): every
 // channel's blocks go through one DCT pass and one double IDCT pass together.
-template <int C, int M0, int M1, int M2, bool FUSE>
+template <int C, int M0, int M1, int M2>
 __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
 {
     using L = FusedLayout<C, M0, M1, M2>;
@@ -1086,7 +1113,7 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
         yv[c] = *reinterpret_cast<const float4 *>(a.y[c] + p);
         gv[c] = *reinterpret_cast<const float4 *>(a.g[c] + p);
     }
-    projection_norms<C, FUSE>(a, ns);
+    projection_norms<C>(a, ns);
     // The coefficients of this thread's row-pass items, loaded now so that their latency
     // overlaps the tile loads (the row pass runs after three barriers).
     constexpr int ITEMS = (TOTB * 8 + NT - 1) / NT;
@@ -1110,8 +1137,8 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
     float yd[C][4];
 #pragma unroll
     for (int c = 0; c < C; c++) {
-        const float norm = proj_norm<FUSE>(a, ns, c);
-        const double rn = proj_rnorm<FUSE>(a, ns, c);
+        const float norm = ns.norm[c];
+        const double rn = ns.rnorm[c];
         const float ys[4] = {yv[c].x, yv[c].y, yv[c].z, yv[c].w};
         const float gs[4] = {gv[c].x, gv[c].y, gv[c].z, gv[c].w};
 #pragma unroll
@@ -1650,7 +1677,7 @@ __constant__ float kDct8[64] = {
     0x1.8f8b84p-4f, -0x1.1c73b4p-2f, 0x1.a9b662p-2f, -0x1.f6297cp-2f, 0x1.f6297cp-2f, -0x1.a9b662p-2f, 0x1.1c73b4p-2f, -0x1.8f8b84p-4f,
 };
 
-template <int C, bool FUSE>
+template <int C>
 __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
 {
     __shared__ NormShared ns;
@@ -1697,12 +1724,12 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
         gl[c] = *reinterpret_cast<const float4 *>(a.g[c] + pix);
         cl[c] = *reinterpret_cast<const int2 *>(a.ch[c].coef + coef_index(c));
     }
-    projection_norms<C, FUSE>(a, ns);
+    projection_norms<C>(a, ns);
 #pragma unroll
     for (int c = 0; c < C; c++) {
         const ChanDev &ch = a.ch[c];
-        const float norm = proj_norm<FUSE>(a, ns, c);
-        const double rn = proj_rnorm<FUSE>(a, ns, c);
+        const float norm = ns.norm[c];
+        const double rn = ns.rnorm[c];
         const float ys[4] = {yl[c].x, yl[c].y, yl[c].z, yl[c].w};
         const float gs[4] = {gl[c].x, gl[c].y, gl[c].z, gl[c].w};
         f32x4 xv;
@@ -1872,20 +1899,43 @@ __device__ __forceinline__ void lane_store_prob(const ChanDev &ch, int by, int b
 // channel's width (valid false) read block 0 / pixel 0, run the transforms on those values
 // within their own lane group, and store nothing.
 
-// Full-resolution channel: 8 lanes per block, block row j = lane >> 3.
-template <bool FUSE>
-__device__ __forceinline__ void lane_unit_full(const IterArgs &a, const NormShared &ns, int c, int by, int bx, int j,
-                                               bool valid)
+// A unit's y, g and coefficient loads.  Full resolution: one block row of 8 pixels (y[0..1],
+// g[0..1]); 4:2:0 chroma: 16 pixels of one full-resolution row (y[0..3], g[0..3]).  An invalid
+// lane loads from offset 0.
+struct LaneLoad {
+    float4 y[4], g[4];
+    int4 raw;
+};
+template <int MODE>
+__device__ __forceinline__ void lane_load(const IterArgs &a, int c, int by, int bx, int r, bool valid, LaneLoad &L)
 {
+    constexpr bool SUB = MODE == PM_SUB22;
+    constexpr int N4 = SUB ? 4 : 2;
     const ChanDev &ch = a.ch[c];
-    const float norm = proj_norm<FUSE>(a, ns, c);
-    const double rn = proj_rnorm<FUSE>(a, ns, c);
+    const int j = SUB ? r >> 1 : r;
     const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
-    const size_t p = valid ? (size_t)(by * 8 + j) * a.W + bx * 8 : 0;
+    const size_t p = !valid ? 0 : (SUB ? (size_t)(by * 16 + r) * a.W + bx * 16 : (size_t)(by * 8 + r) * a.W + bx * 8);
     const float4 *yp = reinterpret_cast<const float4 *>(a.y[c] + p);
     const float4 *gp = reinterpret_cast<const float4 *>(a.g[c] + p);
-    const float4 y0 = yp[0], y1 = yp[1], g0 = gp[0], g1 = gp[1];
-    const int4 raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
+#pragma unroll
+    for (int q4 = 0; q4 < N4; q4++) {
+        L.y[q4] = yp[q4];
+        L.g[q4] = gp[q4];
+    }
+    L.raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
+}
+
+// Full-resolution channel: 8 lanes per block, block row j = lane >> 3.
+__device__ __forceinline__ void lane_unit_full(const IterArgs &a, const NormShared &ns, int c, int by, int bx, int j,
+                                               bool valid, const LaneLoad &L)
+{
+    const ChanDev &ch = a.ch[c];
+    const float norm = ns.norm[c];
+    const double rn = ns.rnorm[c];
+    const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
+    const size_t p = valid ? (size_t)(by * 8 + j) * a.W + bx * 8 : 0;
+    const float4 y0 = L.y[0], y1 = L.y[1], g0 = L.g[0], g1 = L.g[1];
+    const int4 raw = L.raw;
     float q8[8];
     double rq8[8];
 #pragma unroll
@@ -1907,25 +1957,17 @@ __device__ __forceinline__ void lane_unit_full(const IterArgs &a, const NormShar
 // 4:2:0 chroma: 16 lanes per block, full-resolution row r = lane >> 2 of the 16-row band; block
 // row j = r / 2 is held twice (rows 2j and 2j + 1, lanes 4 apart), and the upper row's lane
 // stores the block's outputs.
-template <bool FUSE>
 __device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, const NormShared &ns, int c, int by, int bx, int r,
-                                                bool valid)
+                                                bool valid, const LaneLoad &L)
 {
     const ChanDev &ch = a.ch[c];
-    const float norm = proj_norm<FUSE>(a, ns, c);
-    const double rn = proj_rnorm<FUSE>(a, ns, c);
+    const float norm = ns.norm[c];
+    const double rn = ns.rnorm[c];
     const int j = r >> 1, sy = r & 1;
     const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
     const size_t p = valid ? (size_t)(by * 16 + r) * a.W + bx * 16 : 0;
-    const float4 *yp = reinterpret_cast<const float4 *>(a.y[c] + p);
-    const float4 *gp = reinterpret_cast<const float4 *>(a.g[c] + p);
-    float4 yv[4], gv[4];
-#pragma unroll
-    for (int q4 = 0; q4 < 4; q4++) {
-        yv[q4] = yp[q4];
-        gv[q4] = gp[q4];
-    }
-    const int4 raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
+    const float4 *yv = L.y, *gv = L.g;
+    const int4 raw = L.raw;
     float q8[8];
     double rq8[8];
 #pragma unroll
@@ -1975,27 +2017,34 @@ template <int MODE>
 constexpr int lane_blocks_per_wave() { return MODE == PM_SUB22 ? 4 : 8; }
 
 // One wave per unit = 8 (full resolution) or 4 (4:2:0 chroma) consecutive blocks of a block row;
-// units of channel 0 first.  u0, u1, u2: cumulative unit counts after channels 0, 1, 2.
-template <int C, int M0, int M1, int M2, bool FUSE>
+// units of channel 0 first.  u0, u1, u2: cumulative unit counts after channels 0, 1, 2.  Each
+// workgroup forms the norms once and its waves then take units gridDim.x * NT / 64 apart, so a
+// grid smaller than the unit count spreads that cost over several units per wave.
+template <int C, int M0, int M1, int M2>
 __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t u0, uint32_t u1, uint32_t u2)
 {
     __shared__ NormShared ns;
     const int lane = threadIdx.x & 63;
-    const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
-    projection_norms<C, FUSE, 2>(a, ns);  // before any wave leaves: the fused reduction has barriers
-    if (unit >= u2) return;
-    auto run = [&](auto MODE_C, auto C_C, uint32_t local) {
-        constexpr int MODE = decltype(MODE_C)::value, c = decltype(C_C)::value;
-        constexpr int BPW = lane_blocks_per_wave<MODE>();
-        const ChanDev &ch = a.ch[c];
-        const uint32_t per_row = (ch.bw + BPW - 1) / BPW;
-        const int by = (int)(local / per_row), bx = (int)(local % per_row) * BPW + lane % BPW;
-        if constexpr (MODE == PM_SUB22) lane_unit_sub22<FUSE>(a, ns, c, by, bx, lane / BPW, bx < (int)ch.bw);
-        else lane_unit_full<FUSE>(a, ns, c, by, bx, lane / BPW, bx < (int)ch.bw);
-    };
-    if (unit < u0) run(IC<M0>{}, IC<0>{}, unit);
-    else if (C > 1 && unit < u1) run(IC<M1>{}, IC<1>{}, unit - u0);
-    else if (C > 2) run(IC<M2>{}, IC<2>{}, unit - u1);
+    projection_norms<C>(a, ns);  // before any wave leaves: it has barriers
+    const uint32_t stride = gridDim.x * (NT / 64);
+    for (uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (NT / 64) + (threadIdx.x >> 6)); unit < u2;
+         unit += stride) {
+        auto run = [&](auto MODE_C, auto C_C, uint32_t local) {
+            constexpr int MODE = decltype(MODE_C)::value, c = decltype(C_C)::value;
+            constexpr int BPW = lane_blocks_per_wave<MODE>();
+            const ChanDev &ch = a.ch[c];
+            const uint32_t per_row = (ch.bw + BPW - 1) / BPW;
+            const int by = (int)(local / per_row), bx = (int)(local % per_row) * BPW + lane % BPW;
+            const bool valid = bx < (int)ch.bw;
+            LaneLoad L;
+            lane_load<MODE>(a, c, by, bx, lane / BPW, valid, L);
+            if constexpr (MODE == PM_SUB22) lane_unit_sub22(a, ns, c, by, bx, lane / BPW, valid, L);
+            else lane_unit_full(a, ns, c, by, bx, lane / BPW, valid, L);
+        };
+        if (unit < u0) run(IC<M0>{}, IC<0>{}, unit);
+        else if (C > 1 && unit < u1) run(IC<M1>{}, IC<1>{}, unit - u0);
+        else if (C > 2) run(IC<M2>{}, IC<2>{}, unit - u1);
+    }
 }
 
 }  // namespace
@@ -2012,8 +2061,8 @@ struct artefact_hip_plan {
     int layout = 0;  // LAYOUT_*: which k_project variant runs
     int layout_fused = 0;  // the fused layout this plan's channels allow (LAYOUT_GENERIC if none)
     int lane_project = 0;  // fused layouts: 1 = k_project_lane (no LDS), 0 = k_project_fused
-    int fuse_norm = 0;     // 1: the projection workgroups reduce the norms (no k_finalize launch)
     uint32_t lane_units[3] = {0, 0, 0};  // k_project_lane: cumulative units (one wave's blocks) per channel
+    uint32_t lane_wgs = 0;               // k_project_lane: workgroup cap (0: one unit per wave)
     uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
     float qtab[3][64] = {};                       // quantisation tables per channel, f32 (warm-start features)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
@@ -2028,7 +2077,7 @@ struct artefact_hip_plan {
     int16_t *coef = nullptr;             // sum M_c
     float *quant = nullptr;              // [3][64]
     double *rqq = nullptr;               // [3][64]
-    double *partials = nullptr;
+    long long *acc = nullptr;  // [2][ACC_WORDS]: norm accumulators, by x's buffer index
     float *norms = nullptr;
     double *rnorm = nullptr;
     double *norm_sq = nullptr;
@@ -2112,13 +2161,12 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
         a.g[c] = pl->g + off;
         a.ch[c] = pl->ch[c < pl->C ? c : 0];
     }
-    a.partials = pl->partials;
+    a.acc = pl->acc + (size_t)cur * ACC_WORDS;  // x's buffer index alternates as the iterations do
+    a.acc_next = pl->acc + (size_t)(1 - cur) * ACC_WORDS;
     a.norms = pl->norms;
     a.rnorm = pl->rnorm;
     a.norm_sq = pl->norm_sq;
     a.kg_rows = (uint32_t)pl->rows;
-    a.fuse_norm = pl->fuse_norm ? 1u : 0u;
-    a.nwg_grad = pl->grid_g.x * pl->grid_g.y;
     return a;
 }
 
@@ -2134,18 +2182,9 @@ void launch_gradient_c(const artefact_hip_plan *pl, const IterArgs &a, hipStream
         hipLaunchKernelGGL((k_gradient<C, TGV, true>), pl->grid_g, dim3(NT), 0, s, a);
     else
         hipLaunchKernelGGL((k_gradient<C, TGV>), pl->grid_g, dim3(NT), 0, s, a);
-    if (!a.fuse_norm) {
-        if (pl->precision == ARTEFACT_PRECISION_F32_FAST)
-            hipLaunchKernelGGL((k_finalize<C, true>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials,
-                               a.nwg_grad, pl->norms, pl->rnorm, pl->norm_sq);
-        else
-            hipLaunchKernelGGL((k_finalize<C>), dim3(1), dim3(NF), 0, s, (const double *)pl->partials, a.nwg_grad,
-                               pl->norms, pl->rnorm, pl->norm_sq);
-    }
 }
 
-// Gradient sweep + norm: k_gradient then k_finalize (unless the projection that follows
-// reduces the norms itself, a.fuse_norm).
+// The gradient sweep (its workgroups' sums of g^2 go to the accumulator a.acc).
 void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
     switch (pl->C * 2 + (pl->tgv ? 1 : 0)) {
@@ -2158,60 +2197,56 @@ void launch_gradient(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t
     }
 }
 
-template <bool F>
-void launch_project_f(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
+// The projection kernel for the plan's layout and precision.
+void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
 {
     if (pl->precision == ARTEFACT_PRECISION_BF16X3) {
-        if (pl->C == 1) hipLaunchKernelGGL((k_project_mfma<1, F>), pl->grid_p, dim3(NT), 0, s, a);
-        else if (pl->C == 2) hipLaunchKernelGGL((k_project_mfma<2, F>), pl->grid_p, dim3(NT), 0, s, a);
-        else hipLaunchKernelGGL((k_project_mfma<3, F>), pl->grid_p, dim3(NT), 0, s, a);
+        if (pl->C == 1) hipLaunchKernelGGL((k_project_mfma<1>), pl->grid_p, dim3(NT), 0, s, a);
+        else if (pl->C == 2) hipLaunchKernelGGL((k_project_mfma<2>), pl->grid_p, dim3(NT), 0, s, a);
+        else hipLaunchKernelGGL((k_project_mfma<3>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     }
     if (pl->lane_project && pl->layout != LAYOUT_GENERIC) {
-        const dim3 grid((pl->lane_units[2] + NT / 64 - 1) / (NT / 64));
+        uint32_t wgs = (pl->lane_units[2] + NT / 64 - 1) / (NT / 64);
+        if (pl->lane_wgs && pl->lane_wgs < wgs) wgs = pl->lane_wgs;
+        const dim3 grid(wgs);
         const uint32_t u0 = pl->lane_units[0], u1 = pl->lane_units[1], u2 = pl->lane_units[2];
         switch (pl->layout) {
         case LAYOUT_Y:
-            hipLaunchKernelGGL((k_project_lane<1, PM_FULL, PM_FULL, PM_FULL, F>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            hipLaunchKernelGGL((k_project_lane<1, PM_FULL, PM_FULL, PM_FULL>), grid, dim3(NT), 0, s, a, u0, u1, u2);
             return;
         case LAYOUT_444:
-            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_FULL, PM_FULL, F>), grid, dim3(NT), 0, s, a, u0, u1, u2);
+            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_FULL, PM_FULL>), grid, dim3(NT), 0, s, a, u0, u1, u2);
             return;
         default:
-            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_SUB22, PM_SUB22, F>), grid, dim3(NT), 0, s, a, u0, u1,
+            hipLaunchKernelGGL((k_project_lane<3, PM_FULL, PM_SUB22, PM_SUB22>), grid, dim3(NT), 0, s, a, u0, u1,
                                u2);
             return;
         }
     }
     switch (pl->layout) {
     case LAYOUT_Y:
-        hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL, F>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project_fused<1, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     case LAYOUT_444:
-        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_FULL, PM_FULL, F>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_FULL, PM_FULL>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     case LAYOUT_420:
-        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_SUB22, PM_SUB22, F>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project_fused<3, PM_FULL, PM_SUB22, PM_SUB22>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     default: break;
     }
     if (pl->C == 1)
-        hipLaunchKernelGGL((k_project<1, F>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project<1>), pl->grid_p, dim3(NT), 0, s, a);
     else if (pl->C == 2)
-        hipLaunchKernelGGL((k_project<2, F>), pl->grid_p, dim3(NT), 0, s, a);
+        hipLaunchKernelGGL((k_project<2>), pl->grid_p, dim3(NT), 0, s, a);
     else
-        hipLaunchKernelGGL((k_project<3, F>), pl->grid_p, dim3(NT), 0, s, a);
-}
-
-// The projection kernel for the plan's layout, precision and norm mode (a.fuse_norm).
-void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t s)
-{
-    if (a.fuse_norm) launch_project_f<true>(pl, a, s);
-    else launch_project_f<false>(pl, a, s);
+        hipLaunchKernelGGL((k_project<3>), pl->grid_p, dim3(NT), 0, s, a);
 }
 
 int launch_iterations(artefact_hip_plan *pl, hipStream_t s)
 {
+    hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, s, pl->acc);
     int cur = pl->cur;
     for (uint32_t it = 0; it < pl->iterations; it++) {
         const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == pl->iterations);
@@ -2227,6 +2262,7 @@ int launch_iterations(artefact_hip_plan *pl, hipStream_t s)
 // x_prev are left for the caller to write.
 void launch_init(artefact_hip_plan *pl, hipStream_t s, bool upsample = true)
 {
+    hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, s, pl->acc);
     for (uint32_t c = 0; c < pl->C; c++) {
         const ChanDev &d = pl->ch[c];
         float *x = pl->buf[0] + c * pl->N, *xp = pl->buf[1] + c * pl->N;
@@ -2304,7 +2340,7 @@ int rows_for_workgroups(uint32_t W, uint32_t H, uint64_t wgs)
 
 int pick_rows(int device, uint32_t W, uint32_t H, uint32_t C, bool tgv, bool fast, bool wide)
 {
-    // An override below 2 rows is ignored: the partials are sized for strips of at least 2.
+    // An override below 2 rows is ignored (2 is the tuner's shortest strip).
     if (const char *e = getenv("ARTEFACT_KG_ROWS")) {
         const int r = atoi(e);
         if (r >= 2 && r <= 4096) return r;
@@ -2396,7 +2432,14 @@ void set_layout(artefact_hip_plan *pl)
     // (k_project_fused) on every fused layout (4096x3072 4:4:4: 134 vs 194 us).
     pl->lane_project = 1;
     if (const char *e = getenv("ARTEFACT_PROJECT_KERNEL")) pl->lane_project = strcmp(e, "lds") != 0;
-    if (const char *e = getenv("ARTEFACT_FUSE_NORM")) pl->fuse_norm = atoi(e) != 0;
+    // k_project_lane's grid: three workgroups per CU (each forms the norms once, then its
+    // waves take several units), measured on MI355X faster than one unit per wave (2048x1536
+    // 4:2:0: 29.5 against 35 us, the difference being the per-workgroup norm evaluation).
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, pl->device) != hipSuccess || cus < 1)
+        cus = 256;
+    pl->lane_wgs = 3u * (uint32_t)cus;
+    if (const char *e = getenv("ARTEFACT_LANE_WGS")) pl->lane_wgs = (uint32_t)atoi(e);
     if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
         if (atoi(e)) pl->layout = LAYOUT_GENERIC;
     // Lets callers of the one-shot entry points (artefact_hip_compute / _process, batches) opt
@@ -2425,7 +2468,8 @@ int run_colour(hipStream_t stream, const float *y, const float *cb, const float 
 // from a graph: time_loop) and the winner is cached for the process, per device and geometry,
 // under g_tune_mu.
 struct TuneChoice {
-    int rows, layout, lane, fuse;
+    int rows, layout, lane;
+    uint32_t wgs;  // k_project_lane's workgroup cap
     float ms;
 };
 std::mutex g_tune_mu;
@@ -2464,6 +2508,8 @@ float time_loop(artefact_hip_plan *pl, uint32_t nt)
         launch_init(pl, pl->stream);
         ok = hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
         if (ok) {
+            // Each replay starts from cleared accumulators, as launch_iterations does.
+            hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, pl->stream, pl->acc);
             int cur = 0;
             for (uint32_t it = 0; it < nt; it++) {
                 const IterArgs a = make_args(pl, cur, pl->factors[it], false);
@@ -2500,7 +2546,7 @@ void drop_graph(artefact_hip_plan *pl)
 int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
 {
     const std::string key = tune_key(pl);
-    TuneChoice best{pl->rows, pl->layout, pl->lane_project, pl->fuse_norm, -1.0f};
+    TuneChoice best{pl->rows, pl->layout, pl->lane_project, pl->lane_wgs, -1.0f};
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -2511,7 +2557,7 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
         auto apply = [pl](const TuneChoice &c) {
             pl->layout = c.layout;
             pl->lane_project = c.lane;
-            pl->fuse_norm = c.fuse;
+            pl->lane_wgs = c.wgs;
             set_rows(pl, c.rows);
         };
         const uint32_t nt = std::min<uint32_t>(pl->iterations, 16);
@@ -2554,23 +2600,26 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
                 set_rows(pl, r);
                 const float t = time_loop(pl, nt);
                 if (t < 0.0f) return ARTEFACT_E_HIP;
-                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, pj.first, pj.second, pl->fuse_norm, t};
+                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, pj.first, pj.second, pl->lane_wgs, t};
             }
-        // The winner once more with the norm reduction moved into its projection workgroups
-        // (no k_finalize launch): fewer launches per iteration, but every projection workgroup
-        // reads all k_gradient partials.
-        if (!getenv("ARTEFACT_FUSE_NORM")) {
+        // The lane kernel's workgroup count on the winner: 2-4 per CU, or one unit per wave.
+        if (best.lane && best.layout != LAYOUT_GENERIC && !getenv("ARTEFACT_LANE_WGS")) {
+            int per_cu = 0, cus = 0;
+            gradient_residency(pl->device, pl->C, pl->tgv, false, false, per_cu, cus);
             pl->layout = best.layout;
             pl->lane_project = best.lane;
-            pl->fuse_norm = 1 - best.fuse;
             set_rows(pl, best.rows);
-            const float t = time_loop(pl, nt);
-            if (t < 0.0f) return ARTEFACT_E_HIP;
-            if (t < best.ms) best = TuneChoice{best.rows, best.layout, best.lane, pl->fuse_norm, t};
+            for (uint32_t w : {2u * cus, 4u * cus, 0u}) {
+                if (w == best.wgs) continue;
+                pl->lane_wgs = w;
+                const float t = time_loop(pl, nt);
+                if (t < 0.0f) return ARTEFACT_E_HIP;
+                if (t < best.ms) best = TuneChoice{best.rows, best.layout, best.lane, w, t};
+            }
         }
         // Guard against a noisy pick: time the winner and the default alternately, twice each,
         // and keep the winner only if it is faster both times.
-        if (best.rows != dflt.rows || best.layout != dflt.layout || best.lane != dflt.lane || best.fuse != dflt.fuse) {
+        if (best.rows != dflt.rows || best.layout != dflt.layout || best.lane != dflt.lane || best.wgs != dflt.wgs) {
             int wins = 0;
             float tb = 0.0f, td = 0.0f;
             for (int round = 0; round < 2; round++) {
@@ -2591,7 +2640,7 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
     }
     pl->layout = best.layout;
     pl->lane_project = best.lane;
-    pl->fuse_norm = best.fuse;
+    pl->lane_wgs = best.wgs;
     set_rows(pl, best.rows);
     drop_graph(pl);
     pl->initialized = false;
@@ -2665,9 +2714,6 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     pl->rows = pick_rows(pl->device, pl->W, pl->H, pl->C, pl->tgv, false, pl->wide);
     pl->grid_g = dim3((pl->W + KG_COLS - 1) / KG_COLS, (pl->H + WAVES * pl->rows - 1) / (WAVES * pl->rows));
     pl->grid_p = dim3((pl->W + TW - 1) / TW, (pl->H + TH - 1) / TH);
-    // Partials for the largest grid any strip height (>= 2) can give, so that the tuner can
-    // change the height without reallocating.
-    const size_t nwg = (size_t)pl->grid_g.x * ((pl->H + WAVES * 2 - 1) / (WAVES * 2));
     bool ok = hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking) == hipSuccess;
     // One device allocation holds every buffer, each at a 256-byte boundary: a one-shot call
     // (artefact_hip_compute / _process) pays one hipMalloc and one hipFree, not 13 of each.
@@ -2680,7 +2726,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
     };
     const size_t at_x = take(cn * 4), at_xp = take(cn * 4), at_g = take(cn * 4), at_y = take(cn * 4);
     const size_t at_cos = take(mtot * 4), at_prob = take(mtot * 4), at_coef = take(mtot * 2);
-    const size_t at_quant = take(3 * 64 * 4), at_rqq = take(3 * 64 * 8), at_part = take(nwg * 6 * 8);
+    const size_t at_quant = take(3 * 64 * 4), at_rqq = take(3 * 64 * 8), at_acc = take(2 * ACC_WORDS * 8);
     const size_t at_norms = take(3 * 4), at_rnorm = take(3 * 8), at_nsq = take(3 * 8);
     ok = ok && hipMalloc(&pl->arena, arena_bytes) == hipSuccess;
     // A failed allocation stays the thread's last HIP error until read; it is handled here, so it
@@ -2697,7 +2743,7 @@ int artefact_hip_plan_create(const artefact_params *p, const artefact_channel *c
         pl->coef = reinterpret_cast<int16_t *>(base + at_coef);
         pl->quant = reinterpret_cast<float *>(base + at_quant);
         pl->rqq = reinterpret_cast<double *>(base + at_rqq);
-        pl->partials = reinterpret_cast<double *>(base + at_part);
+        pl->acc = reinterpret_cast<long long *>(base + at_acc);
         pl->norms = reinterpret_cast<float *>(base + at_norms);
         pl->rnorm = reinterpret_cast<double *>(base + at_rnorm);
         pl->norm_sq = reinterpret_cast<double *>(base + at_nsq);
@@ -2852,9 +2898,13 @@ int artefact_hip_plan_first_gradient(artefact_hip_plan *pl, float *const *out)
     if (!pl || !out) return ARTEFACT_E_ARGUMENT;
     if (!pl->initialized) return ARTEFACT_E_STATE;
     DeviceGuard guard(pl->device);
-    IterArgs ga = make_args(pl, pl->cur, 0.0f, false);
-    ga.fuse_norm = 0;  // the norms are part of this call's result
+    const IterArgs ga = make_args(pl, pl->cur, 0.0f, false);
+    // The gradient and its norms (last_norms), from a cleared accumulator.
+    hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, pl->stream, pl->acc);
     launch_gradient(pl, ga, pl->stream);
+    if (pl->C == 1) hipLaunchKernelGGL(k_norms<1>, dim3(1), dim3(NT), 0, pl->stream, ga);
+    else if (pl->C == 2) hipLaunchKernelGGL(k_norms<2>, dim3(1), dim3(NT), 0, pl->stream, ga);
+    else hipLaunchKernelGGL(k_norms<3>, dim3(1), dim3(NT), 0, pl->stream, ga);
     bool ok = hipGetLastError() == hipSuccess;
     for (uint32_t c = 0; c < pl->C; c++) {
         if (!out[c]) return ARTEFACT_E_ARGUMENT;
@@ -2876,6 +2926,7 @@ int artefact_hip_plan_profile(artefact_hip_plan *pl, float *avg_ms)
     for (auto &e : ev) ok = ok && hipEventCreate(&e) == hipSuccess;
     int cur = pl->cur;
     if (ok && n) {
+        hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, pl->stream, pl->acc);
         ok = hipEventRecord(ev[0], pl->stream) == hipSuccess;
         for (uint32_t it = 0; ok && it < n; it++) {
             const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == n);
@@ -3139,15 +3190,13 @@ int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
         hipLaunchKernelGGL(k_upsample_linear, dim3((unsigned)((pl->N + NT - 1) / NT)), dim3(NT), 0, pl->stream,
                            pl->ch[c], (const float *)pl->ch[c].prob, y, pl->W, pl->H);
     }
-    // Projection of the prediction: k_project with zero norms (no descent step) reads y =
-    // prediction and writes the projected image into buf[1] and the prob plane of its clamped
-    // coefficients, the state the loop carries into its next gradient.
-    bool ok = hipMemsetAsync(pl->norms, 0, 3 * sizeof(float), pl->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(pl->rnorm, 0, 3 * sizeof(double), pl->stream) == hipSuccess;
-    IterArgs pa = make_args(pl, 0, 0.0f, false);
-    pa.fuse_norm = 0;  // the zeroed norms, not a reduction of stale partials
+    // Projection of the prediction: the projection kernel with zero norms (the accumulator
+    // launch_init cleared: no descent step) reads y = prediction and writes the projected
+    // image into buf[1] and the prob plane of its clamped coefficients, the state the loop
+    // carries into its next gradient.
+    const IterArgs pa = make_args(pl, 0, 0.0f, false);
     launch_project(pl, pa, pl->stream);
-    ok = ok && hipMemcpyAsync(pl->buf[0], pl->buf[1], (size_t)pl->C * pl->N * sizeof(float),
+    bool ok = hipMemcpyAsync(pl->buf[0], pl->buf[1], (size_t)pl->C * pl->N * sizeof(float),
                               hipMemcpyDeviceToDevice, pl->stream) == hipSuccess;
     ok = ok && hipGetLastError() == hipSuccess;
     pl->cur = 0;
@@ -3159,14 +3208,14 @@ int artefact_hip_plan_tune(artefact_hip_plan *pl, artefact_tuning *out)
 {
     if (!pl) return ARTEFACT_E_ARGUMENT;
     DeviceGuard guard(pl->device);
-    TuneChoice t{pl->rows, pl->layout, pl->lane_project, pl->fuse_norm, -1.0f};
+    TuneChoice t{pl->rows, pl->layout, pl->lane_project, pl->lane_wgs, -1.0f};
     const int rc = tune_plan(pl, &t);
     if (rc) return rc;
     if (out) {
         out->kg_rows = (uint32_t)t.rows;
         out->project_kernel = t.layout == LAYOUT_GENERIC ? 0u : (t.lane ? 2u : 1u);
         out->ms_per_iteration = t.ms;
-        out->fused_norm = (uint32_t)t.fuse;
+        out->project_workgroups = t.wgs;
     }
     return ARTEFACT_OK;
 }
@@ -3338,7 +3387,7 @@ int artefact_hip_compute(const artefact_params *p, const artefact_channel *ch, f
 // The loop without any allocation (artefact_hip_compute_noalloc).  Every buffer beyond the
 // caller's coefficients and output planes -- x and x_prev, the extrapolated image and the
 // gradient, the coefficient estimate and the prob plane, the quantisation tables, the norm
-// partials and norms -- is carved from one static device array, which each device that runs
+// accumulators and norms -- is carved from one static device array, which each device that runs
 // the library holds from the first call until the process ends.  Host code never passes the
 // array's name to a kernel (that is a host-side stand-in, not a device address): the device
 // address comes from hipGetSymbolAddress once per device, under g_scratch_mu.
@@ -3402,9 +3451,6 @@ int artefact_hip_compute_noalloc(const artefact_params *p, const artefact_channe
         pl.M[c] = (size_t)ch[c].rounded_w * ch[c].rounded_h;
         mtot += pl.M[c];
     }
-    // Partials for the smallest strip height (2 rows), so that the size check needs no device query.
-    const size_t nwg = (size_t)((pl.W + KG_COLS - 1) / KG_COLS) * ((pl.H + WAVES * 2 - 1) / (WAVES * 2));
-
     // Scratch layout, each buffer at a 256-byte boundary, as the plan arena.
     const size_t cn = (size_t)pl.C * pl.N;
     size_t bytes = 0;
@@ -3415,7 +3461,7 @@ int artefact_hip_compute_noalloc(const artefact_params *p, const artefact_channe
     };
     const size_t at_x = take(cn * 4), at_xp = take(cn * 4), at_g = take(cn * 4), at_y = take(cn * 4);
     const size_t at_cos = take(mtot * 4), at_prob = take(mtot * 4);
-    const size_t at_quant = take(3 * 64 * 4), at_rqq = take(3 * 64 * 8), at_part = take(nwg * 6 * 8);
+    const size_t at_quant = take(3 * 64 * 4), at_rqq = take(3 * 64 * 8), at_acc = take(2 * ACC_WORDS * 8);
     const size_t at_norms = take(3 * 4), at_rnorm = take(3 * 8), at_nsq = take(3 * 8);
     if (bytes > ARTEFACT_SCRATCH_BYTES) return ARTEFACT_E_SCRATCH;
     if (hipGetDevice(&pl.device) != hipSuccess) return ARTEFACT_E_HIP;
@@ -3434,7 +3480,7 @@ int artefact_hip_compute_noalloc(const artefact_params *p, const artefact_channe
     pl.prob = reinterpret_cast<float *>(base + at_prob);
     pl.quant = reinterpret_cast<float *>(base + at_quant);
     pl.rqq = reinterpret_cast<double *>(base + at_rqq);
-    pl.partials = reinterpret_cast<double *>(base + at_part);
+    pl.acc = reinterpret_cast<long long *>(base + at_acc);
     pl.norms = reinterpret_cast<float *>(base + at_norms);
     pl.rnorm = reinterpret_cast<double *>(base + at_rnorm);
     pl.norm_sq = reinterpret_cast<double *>(base + at_nsq);
@@ -3458,8 +3504,9 @@ int artefact_hip_compute_noalloc(const artefact_params *p, const artefact_channe
     }
     set_layout(&pl);
 
-    // Every buffer the loop reads is written first (tables, start image, cos, prob = 0), so the
-    // previous call's contents need no clearing.
+    // Every buffer the loop reads is written first (tables, start image, cos, prob = 0) or
+    // cleared by a kernel (the norm accumulators, in launch_init), so the previous call's
+    // contents do not matter.
     hipLaunchKernelGGL(k_tables, dim3(1), dim3(64), 0, pl.stream, tables, pl.quant, pl.rqq);
     launch_init(&pl, pl.stream);
     float term = 1.0f;

@@ -141,20 +141,21 @@ int artefact_hip_plan_set_precision(artefact_hip_plan *plan, uint32_t precision)
 /* Autotuner result: the k_gradient strip height (output rows per wave) and the projection
  * kernel (2 = the fused multi-channel kernel without LDS, 1 = the fused kernel with LDS
  * transposes, 0 = the per-channel kernel) chosen for the plan's geometry, the measured
- * milliseconds per iteration, and whether the projection workgroups reduce the gradient
- * norms themselves (1) instead of a separate reduction launch (0). */
+ * milliseconds per iteration, and the workgroup count of the kernel without LDS (each of its
+ * workgroups forms the gradient norms once and then projects several block-row units; 0 =
+ * one unit per wave). */
 typedef struct {
     uint32_t kg_rows;
     uint32_t project_kernel;
     float ms_per_iteration;
-    uint32_t fused_norm;
+    uint32_t project_workgroups;
 } artefact_tuning;
 
-/* Times candidate strip heights and projection kernels on the plan (a few iterations
- * each) and keeps the fastest; the choice is cached per geometry for the process.  Leaves
- * the plan uninitialised (call artefact_hip_plan_init next).  ARTEFACT_KG_ROWS,
- * ARTEFACT_GENERIC_PROJECT and ARTEFACT_FUSE_NORM in the environment pin their parameter.  No reference
- * counterpart: the reference has one fixed schedule. */
+/* Times candidate strip heights, projection kernels and projection workgroup counts on the
+ * plan (a few iterations each) and keeps the fastest; the choice is cached per geometry for the
+ * process.  Leaves the plan uninitialised (call artefact_hip_plan_init next).  ARTEFACT_KG_ROWS,
+ * ARTEFACT_GENERIC_PROJECT, ARTEFACT_PROJECT_KERNEL and ARTEFACT_LANE_WGS in the environment pin
+ * their parameter.  No reference counterpart: the reference has one fixed schedule. */
 int artefact_hip_plan_tune(artefact_hip_plan *plan, artefact_tuning *out);
 
 /* Many images in one hipGraph (the batched form).  Each image is one compute() call's

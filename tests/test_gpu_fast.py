@@ -91,10 +91,10 @@ def test_fast_at_full_size():
         assert dev.max() <= 0.5 + 1e-3, dev.max()
 
 
-def test_fast_with_fused_norm_within_tolerance(monkeypatch):
-    monkeypatch.setenv("ARTEFACT_FUSE_NORM", "1")
+def test_fast_with_one_unit_per_wave_within_tolerance(monkeypatch):
+    monkeypatch.setenv("ARTEFACT_LANE_WGS", "0")
     wl = synth.workload("tgv_420_96x64", seed=22)
-    print("case fast fused norm tgv_420_96x64 iters=20", flush=True)
+    print("case fast one projection unit per wave tgv_420_96x64 iters=20", flush=True)
     fast, f_fast = solve(wl, 20, "f32_fast")
     assert_within_reference_spread(wl, 20, fast, f_fast)
 

@@ -2,13 +2,16 @@
 
 The HIP kernels reproduce the reference's f32 operation order; the only order
 difference is the gradient norm (sequential f32 fold in the reference,
-compute_step.rs:54-58; a once-rounded double-double sum on the GPU).  So:
+compute_step.rs:54-58; on the GPU, per-lane f64 sums merged per workgroup in double-double
+and added exactly in a fixed-point accumulator, rounded once).  So:
 
 * against the oracle in norm_mode="f64" (same norm) the planes must be
   bit-identical (``array_equal`` on the raw bits);
 * against the oracle in the reference's own norm mode they must agree within
   the tolerances stated below (max-abs on the 0..255 scale, relative objective).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -229,10 +232,16 @@ def test_first_gradient_extreme_scales(scale):
     with artefact_amd.Plan(wl["coefs"], 0.3, wl["pweight"], 1, W, H, init_override=planes) as plan:
         plan.init()
         g = plan.first_gradient()
+        norms = plan.last_norms()
     cos = [c.dct_coefs.astype(np.float32) * np.tile(c.quant_table.astype(np.float32), c.block_count) for c in wl["coefs"]]
     gref, _, _ = oracle.gradient(wl["coefs"], 0.3, wl["pweight"], W, H, planes, cos)
     for c, (a, b) in enumerate(zip(g, gref)):
         assert same_bits_or_both_nan(a, b), f"channel {c}"
+    # The norms from the fixed-point accumulator: the square root of the correctly rounded
+    # sum of squares (math.fsum), rounded to f32.
+    for c, b in enumerate(gref):
+        exact = math.fsum((np.asarray(b, np.float64) ** 2).tolist())
+        assert bits_equal(np.float32(np.sqrt(exact)), norms[c]), f"channel {c}: {norms[c]!r} vs {np.sqrt(exact)!r}"
 
 
 @pytest.mark.parametrize("kernel", ["lds", "lane"])
@@ -294,32 +303,51 @@ def test_bit_exact_at_configured_sizes(name, iters):
     ("tgv_422_80x40", 5, {}),
     ("tgv_420_2048", 2, {}),
 ])
-def test_fused_norm_bit_exact(name, iters, env, monkeypatch):
-    """Norms reduced by every projection workgroup (no k_finalize launch) give the same bits
-    as the separate reduction: against the oracle with the f64 norm, for each projection kernel."""
-    monkeypatch.setenv("ARTEFACT_FUSE_NORM", "1")
+def test_projection_kernels_bit_exact(name, iters, env, monkeypatch):
+    """Each projection kernel forms the norms from the accumulator in every workgroup: the
+    same bits as the oracle with the f64 norm."""
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     wl = synth.workload(name, seed=31)
     W, H = wl["max_w"], wl["max_h"]
-    print(f"case fused-norm {name} iters={iters} {env}", flush=True)
+    print(f"case projection kernel {name} iters={iters} {env}", flush=True)
     gpu = run_gpu(wl, iters)
     ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, norm_mode="f64")
     for c, (a, b) in enumerate(zip(gpu, ref)):
         assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
 
 
-def test_fused_norm_precision_form_unchanged(monkeypatch):
-    """The BF16 MFMA projection with the norms reduced in its own workgroups: the same bits as
-    with the separate reduction, and the plan's last norms agree."""
+@pytest.mark.parametrize("wgs", ["0", "1", "7", "40"])
+@pytest.mark.parametrize("name", ["tgv_420_208x144", "tgv_444_72x40", "tv_luma_64x64"])
+def test_projection_workgroup_counts_bit_exact(name, wgs, monkeypatch):
+    """The lane projection kernel with one unit per wave (0) and with 1, 7 or 40 workgroups,
+    whose waves then loop over several units each: the oracle's bits."""
+    monkeypatch.setenv("ARTEFACT_LANE_WGS", wgs)
+    monkeypatch.setenv("ARTEFACT_PROJECT_KERNEL", "lane")
+    wl = synth.workload(name, seed=41)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case projection workgroups {name} wgs={wgs}", flush=True)
+    gpu = run_gpu(wl, 4)
+    ref = oracle.compute(wl["coefs"], wl["weight"], wl["pweight"], 4, W, H, norm_mode="f64")
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16x3"])
+def test_norms_independent_of_strip_height(precision, monkeypatch):
+    """Strips of 2 rows and the default height give k_gradient different workgroup counts and
+    finishing orders; the accumulated norms and the planes keep their bits."""
     wl = synth.workload("tgv_444_72x40", seed=37)
     W, H = wl["max_w"], wl["max_h"]
     outs, norms = [], []
-    for fuse in ("0", "1"):
-        monkeypatch.setenv("ARTEFACT_FUSE_NORM", fuse)
-        print(f"case fused-norm precision fuse={fuse}", flush=True)
+    for rows in ("2", None):
+        if rows:
+            monkeypatch.setenv("ARTEFACT_KG_ROWS", rows)
+        else:
+            monkeypatch.delenv("ARTEFACT_KG_ROWS", raising=False)
+        print(f"case norms {precision} rows={rows}", flush=True)
         with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], 6, W, H, mode=wl["mode"]) as plan:
-            plan.set_precision("bf16x3")
+            plan.set_precision(precision)
             plan.init()
             plan.iterate()
             outs.append(plan.download())
