@@ -25,6 +25,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -34,47 +35,66 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 ITERATION_KERNELS = ("k_gradient", "k_project")
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tgv_420_2048x1536.json")
-TRAFFIC_FILE_FAST = os.path.join(ROOT, "profiles", "traffic_tgv_420_2048x1536_fast.json")
 
 
-def summarize_pmc(fetch_csv, write_csv, copy_bytes=256 << 20):
-    """Per-kernel HBM bytes from two rocprofv3 --pmc CSVs (FETCH_SIZE, WRITE_SIZE, KiB per
-    dispatch), each scaled by what the counter reads for the calibration copy of
-    copy_bytes (the largest __amd_rocclr_copyBuffer dispatch)."""
+def summarize_pmc(read_csv, write_csv, copy_bytes=256 << 20):
+    """Per-kernel L2-to-memory (EA) bytes per dispatch from two rocprofv3 --pmc CSVs, sized by
+    request: reads TCC_EA0_RDREQ_32B/_64B/_128B_sum (32, 64, 128 bytes each; TCC_EA0_RDREQ_sum
+    checks that they cover every request), writes TCC_EA0_WRREQ_sum and TCC_EA0_WRREQ_64B_sum
+    (64-byte requests, the rest 32).  These are L2 misses: what the XCDs' L2s fetch from and
+    write to the memory side (Infinity Cache and HBM).  The calibration copy of copy_bytes (the
+    largest __amd_rocclr_copyBuffer dispatches) is reported as a check, not used as a factor."""
     import csv
     from collections import defaultdict
 
-    def load(path, counter):
-        per, copies = defaultdict(list), []
+    def load(path):
+        per = defaultdict(lambda: defaultdict(list))  # kernel -> counter -> values per dispatch
         for r in csv.DictReader(open(path)):
-            if r["Counter_Name"] != counter:
-                continue
-            v = float(r["Counter_Value"])
-            if "copyBuffer" in r["Kernel_Name"]:
-                copies.append((int(r["Grid_Size"]), v))
+            name = r["Kernel_Name"]
+            if "copyBuffer" in name:
+                short = "copy"
+            elif "k_gradient" in name:
+                short = "k_gradient_fast" if "true, true" in name else "k_gradient"
             else:
-                name = r["Kernel_Name"]
-                short = next((k for k in ITERATION_KERNELS if k in name), name.split("(")[0])
-                per[short].append(v)
-        # The calibration copies are the largest copy dispatches of the run.
-        top = sorted(v for _, v in copies)[-4:]
-        return per, (sum(top) / len(top) if top else None)
+                m = re.search(r"\b(k_\w+)", name)
+                short = m.group(1) if m else "other"
+            per[short][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        return per
 
-    fetch, fcopy = load(fetch_csv, "FETCH_SIZE")
-    write, wcopy = load(write_csv, "WRITE_SIZE")
-    known_kib = copy_bytes / 1024.0
-    fscale = known_kib / fcopy if fcopy else 1.0
-    wscale = known_kib / wcopy if wcopy else 1.0
+    rd, wr = load(read_csv), load(write_csv)
+
+    def mean(d, k, c):
+        v = d.get(k, {}).get(c, [])
+        return sum(v) / len(v) if v else 0.0
+
+    def top_mean(d, c):  # the calibration copies: the largest copy dispatches of the run
+        v = sorted(d.get("copy", {}).get(c, []))[-4:]
+        return sum(v) / len(v) if v else 0.0
+
+    def sized(get):
+        n32, n64, n128 = (get("TCC_EA0_RDREQ_32B_sum"), get("TCC_EA0_RDREQ_64B_sum"),
+                          get("TCC_EA0_RDREQ_128B_sum"))
+        n, w, w64 = get("TCC_EA0_RDREQ_sum"), get("TCC_EA0_WRREQ_sum"), get("TCC_EA0_WRREQ_64B_sum")
+        return {"read_requests": int(n), "read_unsized": int(n - n32 - n64 - n128),
+                "read_bytes": int(32 * n32 + 64 * n64 + 128 * n128),
+                "write_bytes": int(32 * (w - w64) + 64 * w64)}
+
     kernels = {}
-    for k in ITERATION_KERNELS:
-        if k in fetch:
-            fk = sum(fetch[k]) / len(fetch[k])
-            wk = sum(write.get(k, [0.0])) / max(len(write.get(k, [])), 1)
-            kernels[k] = {"fetch_kib": round(fk, 1), "write_kib": round(wk, 1),
-                          "bytes": int(round((fk * fscale + wk * wscale) * 1024))}
-    return {"fetch_scale": round(fscale, 4), "write_scale": round(wscale, 4),
-            "copy_fetch_kib": fcopy, "copy_write_kib": wcopy, "copy_bytes": copy_bytes,
-            "kernels": kernels, "bytes_per_iteration": sum(v["bytes"] for v in kernels.values())}
+    for k in sorted(set(rd) | set(wr)):
+        if k == "copy":
+            continue
+        r = sized(lambda c: mean(rd, k, c) if c.startswith("TCC_EA0_RD") else mean(wr, k, c))
+        r["bytes"] = r["read_bytes"] + r["write_bytes"]
+        kernels[k] = r
+    copy = sized(lambda c: top_mean(rd, c) if c.startswith("TCC_EA0_RD") else top_mean(wr, c))
+    copy["expected_bytes_each_way"] = copy_bytes
+    def total(names):
+        return sum(kernels[k]["bytes"] for k in names if k in kernels)
+
+    return {"counters": "L2-to-EA requests by size (TCC_EA0_RDREQ_32B/64B/128B, TCC_EA0_WRREQ/_64B)",
+            "copy_check": copy, "kernels": kernels,
+            "bytes_per_iteration": total(("k_gradient", "k_project_lane")),
+            "bytes_per_iteration_fast": total(("k_gradient_fast", "k_project_lane"))}
 
 
 def algorithmic_bytes(coefs, W, H):
@@ -883,8 +903,9 @@ def main() -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the other configurations (512x512 TV, precision form, batch, warm start) of the "
                          "default line")
-    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="print per-kernel HBM bytes from two rocprofv3 --pmc CSVs and exit")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("READ_CSV", "WRITE_CSV"),
+                    help="print per-kernel L2-to-memory bytes from two rocprofv3 --pmc CSVs (request-size "
+                         "counters) and exit")
     args = ap.parse_args()
     if args.summarize_pmc:
         print(json.dumps(summarize_pmc(*args.summarize_pmc), indent=1))
@@ -963,10 +984,10 @@ def main() -> int:
     if calib is not None and args.copy_sweep:
         calib["copy_GBps_by_size_MiB"] = {mb: copy_calibration(mb << 20, reps=8)["copy_GBps"] for mb in (16, 64, 128)}
     traffic = None
-    tfile = TRAFFIC_FILE_FAST if args.precision == "f32_fast" else TRAFFIC_FILE
-    if os.path.exists(tfile) and args.workload == "tgv_420_2048":
-        with open(tfile) as fh:
-            traffic = json.load(fh).get("bytes_per_iteration")
+    if os.path.exists(TRAFFIC_FILE) and args.workload == "tgv_420_2048":
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh).get("bytes_per_iteration_fast" if args.precision == "f32_fast"
+                                        else "bytes_per_iteration")
 
     res = {
         "metric": "Mpixel*iterations/s + time to plain-100-iter objective",
