@@ -1311,30 +1311,41 @@ __global__ __launch_bounds__(NT) void k_init_upsample(const ChanDev ch, const fl
     xp[i] = v;
 }
 
-// Bilinear upsampling of a component-resolution plane onto the grid, for the learned start
+// Bilinear upsampling of component-resolution planes onto the grid, for the learned start
 // image: component sample i sits at grid coordinate (i + 0.5) * s - 0.5, and the edges clamp.
 // Without subsampling (s = 1) every weight is 0 or 1 and the plane is copied.  Replication
-// (k_init_upsample) would leave a TV jump every s pixels in a smooth prediction.
-__global__ __launch_bounds__(NT) void k_upsample_linear(const ChanDev ch, const float *plane, float *out,
-                                                         uint32_t W, uint32_t H)
+// (k_init_upsample) would leave a TV jump every s pixels in a smooth prediction.  One launch
+// for all channels (blockIdx.z); a workgroup covers NT pixels of a row, and the rows stride by
+// gridDim.y.
+struct UpsampleArgs {
+    const float *plane[3];
+    float *out[3];
+    uint32_t rw[3], rh[3], hs[3], vs[3];
+    uint32_t W, H;
+};
+__global__ __launch_bounds__(NT) void k_upsample_linear(const UpsampleArgs u)
 {
-    const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
-    if (i >= (size_t)W * H) return;
-    const uint32_t y = i / W, xx = i % W;
-    const float u = ((float)xx + 0.5f) / (float)ch.hs - 0.5f;
-    const float v = ((float)y + 0.5f) / (float)ch.vs - 0.5f;
-    const float fu = floorf(u), fv = floorf(v);
-    const float au = u - fu, av = v - fv;
+    const uint32_t c = blockIdx.z, xx = blockIdx.x * NT + threadIdx.x;
+    if (xx >= u.W) return;
+    const uint32_t rw = u.rw[c], rh = u.rh[c];
     auto clamp = [](float t, uint32_t n) {
         const int k = (int)t;
         return (uint32_t)(k < 0 ? 0 : (k > (int)n - 1 ? (int)n - 1 : k));
     };
-    const uint32_t x0 = clamp(fu, ch.rw), x1 = clamp(fu + 1.0f, ch.rw);
-    const uint32_t y0 = clamp(fv, ch.rh), y1 = clamp(fv + 1.0f, ch.rh);
-    const float *r0 = plane + (size_t)y0 * ch.rw, *r1 = plane + (size_t)y1 * ch.rw;
-    const float top = au == 0.0f ? r0[x0] : (1.0f - au) * r0[x0] + au * r0[x1];
-    const float bot = au == 0.0f ? r1[x0] : (1.0f - au) * r1[x0] + au * r1[x1];
-    out[i] = av == 0.0f ? top : (1.0f - av) * top + av * bot;
+    const float uu = ((float)xx + 0.5f) / (float)u.hs[c] - 0.5f;
+    const float fu = floorf(uu), au = uu - fu;
+    const uint32_t x0 = clamp(fu, rw), x1 = clamp(fu + 1.0f, rw);
+    const float *plane = u.plane[c];
+    float *out = u.out[c];
+    for (uint32_t y = blockIdx.y; y < u.H; y += gridDim.y) {
+        const float v = ((float)y + 0.5f) / (float)u.vs[c] - 0.5f;
+        const float fv = floorf(v), av = v - fv;
+        const uint32_t y0 = clamp(fv, rh), y1 = clamp(fv + 1.0f, rh);
+        const float *r0 = plane + (size_t)y0 * rw, *r1 = plane + (size_t)y1 * rw;
+        const float top = au == 0.0f ? r0[x0] : (1.0f - au) * r0[x0] + au * r0[x1];
+        const float bot = au == 0.0f ? r1[x0] : (1.0f - au) * r1[x0] + au * r1[x1];
+        out[(size_t)y * u.W + xx] = av == 0.0f ? top : (1.0f - av) * top + av * bot;
+    }
 }
 
 // cos = c * q (coef.rs:94-104); the prob term of the first iteration is 0 (cos == c q).
@@ -3185,11 +3196,19 @@ int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
         const int rc = cnn_predict_planes(net, planes, quant, n, pl->ch[c0].rh, pl->ch[c0].rw, pred, pl->stream);
         if (rc) return rc;
     }
+    UpsampleArgs ua{};
     for (uint32_t c = 0; c < pl->C; c++) {
-        float *y = pl->ybuf + (size_t)c * pl->N;
-        hipLaunchKernelGGL(k_upsample_linear, dim3((unsigned)((pl->N + NT - 1) / NT)), dim3(NT), 0, pl->stream,
-                           pl->ch[c], (const float *)pl->ch[c].prob, y, pl->W, pl->H);
+        ua.plane[c] = pl->ch[c].prob;
+        ua.out[c] = pl->ybuf + (size_t)c * pl->N;
+        ua.rw[c] = pl->ch[c].rw;
+        ua.rh[c] = pl->ch[c].rh;
+        ua.hs[c] = pl->ch[c].hs;
+        ua.vs[c] = pl->ch[c].vs;
     }
+    ua.W = pl->W;
+    ua.H = pl->H;
+    hipLaunchKernelGGL(k_upsample_linear, dim3((pl->W + NT - 1) / NT, std::min<uint32_t>(pl->H, 65535u), pl->C),
+                       dim3(NT), 0, pl->stream, ua);
     // Projection of the prediction: the projection kernel with zero norms (the accumulator
     // launch_init cleared: no descent step) reads y = prediction and writes the projected
     // image into buf[1] and the prob plane of its clamped coefficients, the state the loop
