@@ -134,6 +134,7 @@ SIGNATURES = [
      [ctypes.POINTER(artefact_batch_image), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
       ctypes.POINTER(ctypes.c_void_p)]),
     ("artefact_hip_batch_run", ctypes.c_int, [ctypes.c_void_p]),
+    ("artefact_hip_batch_streams", ctypes.c_uint32, [ctypes.c_void_p]),
     ("artefact_hip_batch_synchronize", ctypes.c_int, [ctypes.c_void_p]),
     ("artefact_hip_batch_download", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, FLOAT_PP]),
     ("artefact_hip_batch_plan", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_void_p)]),

@@ -281,8 +281,8 @@ def measure_plan(wl, precision="f32", steps=5, warmup=2):
 
 
 def measure_batch(imgs, steps=5, warmup=2, tune=True):
-    """The images in one graph (4 streams) against the same images solved one after another,
-    each as its own graph-captured plan."""
+    """The images in one graph (stream count autotuned) against the same images solved one after
+    another, each as its own graph-captured plan."""
     import artefact_amd
 
     work = sum(w["max_w"] * w["max_h"] * w["iterations"] for w in imgs)
@@ -290,8 +290,9 @@ def measure_batch(imgs, steps=5, warmup=2, tune=True):
     images = [artefact_amd.BatchImage(w["coefs"], w["weight"], w["pweight"], w["iterations"], w["max_w"], w["max_h"],
                                       w["mode"]) for w in imgs]
     t0 = time.perf_counter()
-    batch = artefact_amd.Batch(images, streams=4, tune=tune)
+    batch = artefact_amd.Batch(images, streams=0, tune=tune)
     setup_s = time.perf_counter() - t0
+    streams = batch.streams
     for _ in range(warmup):
         batch.run()
     batch.synchronize()
@@ -319,7 +320,7 @@ def measure_batch(imgs, steps=5, warmup=2, tune=True):
     for pl in plans:
         pl.close()
     achieved = B / (ms * 1e-3) / 1e9
-    return {"images": len(imgs), "pixel_iterations": work, "ms_per_batch": round(ms, 3),
+    return {"images": len(imgs), "streams": streams, "pixel_iterations": work, "ms_per_batch": round(ms, 3),
             "Mpixel_it_s": round(work / (ms * 1e-3) / 1e6, 2), "sequential_ms": round(seq_ms, 3),
             "batch_speedup_vs_sequential": round(seq_ms / ms, 3), "setup_s": round(setup_s, 2),
             "roofline": {"achieved_GBps": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -467,7 +468,7 @@ def other_configs(args, pre):
 
     b = measure_batch(pre["batch64"], max(2, steps // 2), warmup)
     out["batch64"] = {"workload": "64 images, sides uniform in 256..2048 (multiples of 16), half 4:2:0 / half 4:4:4, "
-                                  "IJG q30..90, half TV / half TGV, 100 iterations, one hipGraph over 4 streams, "
+                                  "IJG q30..90, half TV / half TGV, 100 iterations, one hipGraph over autotuned streams, "
                                   "autotuned per geometry, seed 0", **b}
     print(f"bench: batch64 {b['ms_per_batch']:.1f} ms per batch", file=sys.stderr, flush=True)
 
@@ -811,8 +812,9 @@ def bench_batch(args) -> int:
     images = [artefact_amd.BatchImage(w["coefs"], w["weight"], w["pweight"], w["iterations"], w["max_w"], w["max_h"],
                                       w["mode"]) for w in imgs]
     t0 = time.perf_counter()
-    batch = artefact_amd.Batch(images, streams=4, tune=True)
+    batch = artefact_amd.Batch(images, streams=0, tune=True)
     setup_s = time.perf_counter() - t0
+    streams = batch.streams
     for _ in range(args.warmup):
         batch.run()
     batch.synchronize()
@@ -858,8 +860,8 @@ def bench_batch(args) -> int:
         "data": "synthetic (seeded generator, artefact_amd/synth.py batch_workload)",
         "config": {
             "workload": f"batched form: {count} images, sides uniform in 256..2048 (multiples of 16), half 4:2:0 / "
-                        f"half 4:4:4, IJG quality 30..90, half TV / half TGV, 100 iterations, one hipGraph over 4 "
-                        f"streams, autotuned per geometry, seed 0",
+                        f"half 4:4:4, IJG quality 30..90, half TV / half TGV, 100 iterations, one hipGraph over "
+                        f"{streams} streams (count autotuned), autotuned per geometry, seed 0",
             "images": count,
             "pixel_iterations": work,
             "sequential_ms_per_step": round(seq_ms, 3),

@@ -160,9 +160,10 @@ int artefact_hip_plan_tune(artefact_hip_plan *plan, artefact_tuning *out);
 
 /* Many images in one hipGraph (the batched form).  Each image is one compute() call's
  * arguments (init_override must be NULL).  The graph holds every image's start image and
- * iteration loop; images are spread over `streams` HIP streams (0: 4), largest first onto
- * the least loaded, so independent solves overlap on the GPU.  With `tune` set every
- * geometry is autotuned first.  No reference counterpart: the reference processes one
+ * iteration loop; images are spread over `streams` HIP streams, largest first onto the least
+ * loaded, so independent solves overlap on the GPU.  With `tune` set every geometry is
+ * autotuned first, and `streams` = 0 times the whole graph with 1 to 4 streams and keeps the
+ * fastest; without `tune`, 0 means 2.  No reference counterpart: the reference processes one
  * image per call (lib.rs:99-157). */
 typedef struct {
     const artefact_params *params;
@@ -172,6 +173,8 @@ typedef struct artefact_hip_batch artefact_hip_batch;
 
 int artefact_hip_batch_create(const artefact_batch_image *images, uint32_t count, uint32_t streams, uint32_t tune,
                               artefact_hip_batch **out_batch);
+/* The number of streams the batch's graph spreads its images over (0 for a null batch). */
+uint32_t artefact_hip_batch_streams(const artefact_hip_batch *batch);
 /* Launches the graph (asynchronous): every image is solved from its coefficients. */
 int artefact_hip_batch_run(artefact_hip_batch *batch);
 int artefact_hip_batch_synchronize(artefact_hip_batch *batch);

@@ -31,6 +31,24 @@ def test_batch_matches_oracle_on_every_replay():
                     assert bits_equal(a, b), f"replay {rep} image {w['name']} channel {c}"
 
 
+def test_autotuned_stream_count_matches_oracle():
+    """streams = 0 with tuning: the graph is timed with 1 to 4 streams and the fastest kept;
+    the images still come out bit-exact."""
+    imgs = synth.batch_workload(6, seed=6, min_side=32, max_side=128, iterations=4)
+    print("case batch of 6, autotuned streams", flush=True)
+    with artefact_amd.Batch([artefact_amd.BatchImage(w["coefs"], w["weight"], w["pweight"], w["iterations"],
+                                                     w["max_w"], w["max_h"], w["mode"]) for w in imgs],
+                            streams=0, tune=True) as batch:
+        assert 1 <= batch.streams <= 4
+        for rep in range(2):
+            batch.run()
+            for i, w in enumerate(imgs):
+                ref = oracle.compute(w["coefs"], w["weight"], w["pweight"], w["iterations"], w["max_w"], w["max_h"],
+                                     norm_mode="f64")
+                for a, b in zip(batch.download(i), ref):
+                    assert bits_equal(a, b), f"replay {rep} {w['name']}"
+
+
 def test_single_stream_batch_matches_oracle():
     imgs = synth.batch_workload(3, seed=4, min_side=48, max_side=96, iterations=3)
     print("case batch of 3, 1 stream", flush=True)
