@@ -33,6 +33,7 @@ ARTEFACT_E_SCRATCH = -7
 ARTEFACT_IMAGE_PNG = 0
 ARTEFACT_IMAGE_BMP = 1
 ARTEFACT_IMAGE_TIFF = 2
+ARTEFACT_IMAGE_WEBP = 3
 ARTEFACT_SCRATCH_BYTES = 64 << 20
 
 

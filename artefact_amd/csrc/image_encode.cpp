@@ -14,14 +14,22 @@
  *        4 bytes, uncompressed.
  *   TIFF (TIFF 6.0 baseline RGB, little-endian): one uncompressed strip, chunky RGB, 8 bits
  *        per sample, 72 dpi.
+ *   WebP (lossless, VP8L bitstream of RFC 9649, in a RIFF container): the subtract-green
+ *        transform, no colour cache, no backward references; one prefix code per channel
+ *        (Huffman code lengths from the image's histograms, limited to 15 bits, written as
+ *        a "simple" code for one or two symbols, else a "normal" code whose code-length code
+ *        uses zero runs).  The image crate's WebP encoder is lossless too.
  *
  * Host C++; nothing here touches the GPU.
  */
 #include <zlib.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <queue>
+#include <utility>
 #include <vector>
 
 #include "../../include/artefact_hip.h"
@@ -255,6 +263,245 @@ int encode_tiff(const uint8_t *rgb, uint32_t w, uint32_t h, Writer &out)
     return out.ok ? ARTEFACT_OK : ARTEFACT_E_ARGUMENT;
 }
 
+
+// ---- WebP lossless (VP8L) ----
+
+// Bits in the order VP8L reads them: least significant first.
+struct BitWriter {
+    std::vector<uint8_t> buf;
+    uint64_t acc = 0;
+    int nbits = 0;
+    void put(uint32_t v, int n)
+    {
+        acc |= (uint64_t)v << nbits;
+        nbits += n;
+        while (nbits >= 8) {
+            buf.push_back((uint8_t)acc);
+            acc >>= 8;
+            nbits -= 8;
+        }
+    }
+    void flush()
+    {
+        if (nbits > 0) buf.push_back((uint8_t)acc);
+        acc = 0;
+        nbits = 0;
+    }
+};
+
+// Huffman code lengths for the counts, none longer than max_len; unused symbols get 0.  Ties
+// break on the node index, so the lengths are deterministic.  A tree deeper than max_len is
+// rebuilt from halved counts until it fits (equal counts give a balanced tree).
+std::vector<uint8_t> huffman_lengths(const std::vector<uint64_t> &count, int max_len)
+{
+    const size_t n = count.size();
+    std::vector<uint8_t> len(n, 0);
+    std::vector<uint64_t> c(count);
+    for (;;) {
+        std::vector<std::pair<int, int>> kids;  // internal node k: children (node ids)
+        std::vector<int> leaf_symbol;           // node id < leaves: the symbol
+        using Item = std::pair<uint64_t, int>;  // (weight, node id)
+        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+        for (size_t s = 0; s < n; s++)
+            if (c[s]) {
+                heap.push({c[s], (int)leaf_symbol.size()});
+                leaf_symbol.push_back((int)s);
+            }
+        const int leaves = (int)leaf_symbol.size();
+        if (leaves == 0) return len;
+        if (leaves == 1) {
+            len[leaf_symbol[0]] = 1;
+            return len;
+        }
+        while (heap.size() > 1) {
+            const Item a = heap.top();
+            heap.pop();
+            const Item b = heap.top();
+            heap.pop();
+            kids.push_back({a.second, b.second});
+            heap.push({a.first + b.first, leaves + (int)kids.size() - 1});
+        }
+        // Depths from the root down.
+        std::vector<int> depth(leaves + kids.size(), 0);
+        int deepest = 0;
+        for (int k = (int)kids.size() - 1; k >= 0; k--) {
+            const int d = depth[leaves + k] + 1;
+            depth[kids[k].first] = d;
+            depth[kids[k].second] = d;
+        }
+        for (int i = 0; i < leaves; i++) deepest = std::max(deepest, depth[i]);
+        if (deepest <= max_len) {
+            for (int i = 0; i < leaves; i++) len[leaf_symbol[i]] = (uint8_t)depth[i];
+            return len;
+        }
+        for (uint64_t &v : c)
+            if (v) v = (v >> 1) | 1;
+    }
+}
+
+// Canonical codes for the lengths (shorter codes first, then by symbol), each bit-reversed so
+// that writing it least significant bit first puts its first bit first.
+std::vector<uint32_t> canonical_codes(const std::vector<uint8_t> &len)
+{
+    uint32_t count[16] = {}, next[16] = {};
+    for (uint8_t l : len)
+        if (l) count[l]++;
+    uint32_t code = 0;
+    for (int l = 1; l < 16; l++) {
+        code = (code + count[l - 1]) << 1;
+        next[l] = code;
+    }
+    std::vector<uint32_t> out(len.size(), 0);
+    for (size_t s = 0; s < len.size(); s++) {
+        const int l = len[s];
+        if (!l) continue;
+        const uint32_t c = next[l]++;
+        uint32_t r = 0;
+        for (int i = 0; i < l; i++) r |= ((c >> i) & 1u) << (l - 1 - i);
+        out[s] = r;
+    }
+    return out;
+}
+
+struct PrefixCode {
+    std::vector<uint8_t> len;
+    std::vector<uint32_t> code;
+    void emit(BitWriter &bw, int s) const { bw.put(code[s], len[s]); }
+};
+
+// Writes the prefix code for an alphabet with these symbol counts and returns it.
+PrefixCode write_prefix_code(BitWriter &bw, const std::vector<uint64_t> &count)
+{
+    const size_t n = count.size();
+    PrefixCode p;
+    p.len.assign(n, 0);
+    p.code.assign(n, 0);
+    std::vector<int> used;
+    for (size_t s = 0; s < n; s++)
+        if (count[s]) used.push_back((int)s);
+    if (used.size() <= 2 && (used.empty() || used.back() < 256)) {
+        // Simple code: one symbol (read with no bits) or two (one bit each, the smaller first).
+        if (used.empty()) used.push_back(0);
+        bw.put(1, 1);
+        bw.put((uint32_t)used.size() - 1, 1);
+        if (used[0] < 2) {
+            bw.put(0, 1);
+            bw.put((uint32_t)used[0], 1);
+        } else {
+            bw.put(1, 1);
+            bw.put((uint32_t)used[0], 8);
+        }
+        if (used.size() == 2) {
+            bw.put((uint32_t)used[1], 8);
+            p.len[used[0]] = p.len[used[1]] = 1;
+            p.code[used[1]] = 1;
+        }
+        return p;
+    }
+    p.len = huffman_lengths(count, 15);
+    p.code = canonical_codes(p.len);
+    // The code lengths as tokens: a length 0-15, or a run of zeros (17: 3-10, 18: 11-138).
+    std::vector<std::pair<int, int>> tokens;  // (symbol, extra bits value)
+    for (size_t i = 0; i < n;) {
+        if (p.len[i]) {
+            tokens.push_back({p.len[i], 0});
+            i++;
+            continue;
+        }
+        size_t run = 1;
+        while (i + run < n && !p.len[i + run]) run++;
+        i += run;
+        while (run >= 11) {
+            const size_t k = std::min<size_t>(run, 138);
+            tokens.push_back({18, (int)(k - 11)});
+            run -= k;
+        }
+        if (run >= 3) {
+            tokens.push_back({17, (int)(run - 3)});
+            run = 0;
+        }
+        for (; run; run--) tokens.push_back({0, 0});
+    }
+    std::vector<uint64_t> hist(19, 0);
+    for (const auto &t : tokens) hist[t.first]++;
+    // A code-length code of one symbol would be a one-symbol normal code; a second symbol that
+    // is never written keeps it a complete two-symbol code.
+    int distinct = 0;
+    for (uint64_t v : hist) distinct += v ? 1 : 0;
+    if (distinct == 1) hist[hist[0] ? 1 : 0] = 1;
+    const std::vector<uint8_t> cl_len = huffman_lengths(hist, 7);
+    const std::vector<uint32_t> cl_code = canonical_codes(cl_len);
+    static const int order[19] = {17, 18, 0, 1, 2, 3, 4, 5, 16, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+    int num = 19;
+    while (num > 4 && !cl_len[order[num - 1]]) num--;
+    bw.put(0, 1);  // normal code
+    bw.put((uint32_t)(num - 4), 4);
+    for (int i = 0; i < num; i++) bw.put(cl_len[order[i]], 3);
+    bw.put(0, 1);  // the code lengths of the whole alphabet follow
+    for (const auto &t : tokens) {
+        bw.put(cl_code[t.first], cl_len[t.first]);
+        if (t.first == 17) bw.put((uint32_t)t.second, 3);
+        else if (t.first == 18) bw.put((uint32_t)t.second, 7);
+    }
+    return p;
+}
+
+constexpr uint32_t WEBP_MAX_SIDE = 16384;  // 14-bit width - 1 and height - 1
+
+size_t webp_bound(uint32_t w, uint32_t h)
+{
+    // At most 15 bits for each of green, red and blue per pixel (alpha is one symbol: no bits),
+    // plus headers and the five code descriptions.
+    return (size_t)w * h * 6 + 8192;
+}
+
+int encode_webp(const uint8_t *rgb, uint32_t w, uint32_t h, Writer &out)
+{
+    if (w > WEBP_MAX_SIDE || h > WEBP_MAX_SIDE) return ARTEFACT_E_ARGUMENT;
+    const size_t npx = (size_t)w * h;
+    // Subtract-green transform: red and blue minus green, modulo 256.
+    std::vector<uint64_t> hg(256 + 24, 0), hr(256, 0), hb(256, 0), ha(256, 0), hd(40, 0);
+    for (size_t i = 0; i < npx; i++) {
+        const uint8_t r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        hg[g]++;
+        hr[(uint8_t)(r - g)]++;
+        hb[(uint8_t)(b - g)]++;
+    }
+    ha[255] = npx;
+    BitWriter bw;
+    bw.put(0x2f, 8);  // VP8L signature
+    bw.put(w - 1, 14);
+    bw.put(h - 1, 14);
+    bw.put(0, 1);  // alpha_is_used: every alpha is 255
+    bw.put(0, 3);  // version 0
+    bw.put(1, 1);  // a transform follows:
+    bw.put(2, 2);  // subtract green
+    bw.put(0, 1);  // no further transform
+    bw.put(0, 1);  // no colour cache
+    bw.put(0, 1);  // no meta prefix codes
+    const PrefixCode cg = write_prefix_code(bw, hg), cr = write_prefix_code(bw, hr), cb = write_prefix_code(bw, hb),
+                     ca = write_prefix_code(bw, ha);
+    (void)write_prefix_code(bw, hd);  // distances: no backward references are written
+    for (size_t i = 0; i < npx; i++) {
+        const uint8_t r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        cg.emit(bw, g);
+        cr.emit(bw, (uint8_t)(r - g));
+        cb.emit(bw, (uint8_t)(b - g));
+        ca.emit(bw, 255);
+    }
+    bw.flush();
+    const size_t data = bw.buf.size(), padded = data + (data & 1);
+    if (12 + 8 + padded > 0xFFFFFFFFull) return ARTEFACT_E_ARGUMENT;
+    out.bytes("RIFF", 4);
+    out.le32((uint32_t)(4 + 8 + padded));
+    out.bytes("WEBP", 4);
+    out.bytes("VP8L", 4);
+    out.le32((uint32_t)data);
+    out.bytes(bw.buf.data(), data);
+    if (data & 1) out.u8(0);
+    return out.ok ? ARTEFACT_OK : ARTEFACT_E_ARGUMENT;
+}
+
 }  // namespace
 
 extern "C" {
@@ -266,6 +513,7 @@ size_t artefact_image_encode_bound(uint32_t w, uint32_t h, uint32_t format)
     case ARTEFACT_IMAGE_PNG: return png_bound(w, h);
     case ARTEFACT_IMAGE_BMP: return 14 + 40 + bmp_row(w) * h;
     case ARTEFACT_IMAGE_TIFF: return TIFF_HEADER + 3ull * w * h;
+    case ARTEFACT_IMAGE_WEBP: return w > WEBP_MAX_SIDE || h > WEBP_MAX_SIDE ? 0 : webp_bound(w, h);
     default: return 0;
     }
 }
@@ -282,6 +530,7 @@ int artefact_image_encode(const uint8_t *rgb, uint32_t w, uint32_t h, uint32_t f
         case ARTEFACT_IMAGE_PNG: rc = encode_png(rgb, w, h, wr); break;
         case ARTEFACT_IMAGE_BMP: rc = encode_bmp(rgb, w, h, wr); break;
         case ARTEFACT_IMAGE_TIFF: rc = encode_tiff(rgb, w, h, wr); break;
+        case ARTEFACT_IMAGE_WEBP: rc = encode_webp(rgb, w, h, wr); break;
         default: return ARTEFACT_E_ARGUMENT;
         }
     } catch (const std::bad_alloc &) {

@@ -4,8 +4,8 @@
 path and format (backend/artefact-cli/main.rs:57-98) and saves with the ``image`` crate
 (``img.save(output)``, main.rs:156), which chooses the encoder from the file extension.
 ``output_path`` restates that choice, ``save`` the save, ``encode`` the encoder call
-(csrc/image_encode.cpp: PNG, BMP and TIFF; WebP and GIF, which the CLI also names, are not
-built and raise).
+(csrc/image_encode.cpp: PNG, BMP, TIFF and lossless WebP; GIF, which the CLI also names, is
+not built and raises).
 """
 from __future__ import annotations
 
@@ -16,7 +16,8 @@ import numpy as np
 
 from . import _lib
 
-FORMATS = {"png": _lib.ARTEFACT_IMAGE_PNG, "bmp": _lib.ARTEFACT_IMAGE_BMP, "tiff": _lib.ARTEFACT_IMAGE_TIFF}
+FORMATS = {"png": _lib.ARTEFACT_IMAGE_PNG, "bmp": _lib.ARTEFACT_IMAGE_BMP, "tiff": _lib.ARTEFACT_IMAGE_TIFF,
+           "webp": _lib.ARTEFACT_IMAGE_WEBP}
 # The CLI's list (main.rs:52) and the extensions the image crate maps to a format.
 POSSIBLE_FORMATS = ["png", "webp", "tiff", "bmp", "gif"]
 EXTENSIONS = {"png": "png", "bmp": "bmp", "tif": "tiff", "tiff": "tiff", "webp": "webp", "gif": "gif"}
@@ -27,7 +28,7 @@ class ImageFormatError(ValueError):
 
 
 def encode(rgb: np.ndarray, format: str = "png") -> bytes:
-    """File bytes of a uint8 (height, width, 3) RGB image in ``format`` (png, bmp or tiff)."""
+    """File bytes of a uint8 (height, width, 3) RGB image in ``format`` (png, bmp, tiff or webp)."""
     fmt = FORMATS.get(format.lower())
     if fmt is None:
         raise ImageFormatError(f"cannot encode {format!r}: this build writes {', '.join(FORMATS)}")
@@ -65,7 +66,7 @@ def output_path(input_path: str, output: str | None = None, format: str = "auto"
 
 
 def save(rgb: np.ndarray, path: str) -> None:
-    """Writes the image in the format its extension names (png, bmp, tif / tiff)."""
+    """Writes the image in the format its extension names (png, bmp, tif / tiff, webp)."""
     ext = os.path.splitext(path)[1][1:].lower()
     fmt = EXTENSIONS.get(ext)
     if fmt is None:

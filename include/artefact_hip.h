@@ -279,10 +279,12 @@ void artefact_jpeg_free(artefact_jpeg *jpeg);
  * (backend/artefact-cli/main.rs:156 img.save(output); backend/artefact-wasm/lib.rs:41
  * write_to(.., ImageFormat)), in the formats its features enable (png, bmp, tiff, webp;
  * backend/artefact-lib/Cargo.toml:7-12).  Built here: PNG (8-bit RGB, adaptive filters,
- * deflate), BMP (24-bit) and TIFF (baseline RGB, uncompressed).  Host code. */
+ * deflate), BMP (24-bit), TIFF (baseline RGB, uncompressed) and WebP (lossless VP8L, as the
+ * image crate writes it; sides up to 16384).  Host code. */
 #define ARTEFACT_IMAGE_PNG  0
 #define ARTEFACT_IMAGE_BMP  1
 #define ARTEFACT_IMAGE_TIFF 2
+#define ARTEFACT_IMAGE_WEBP 3
 /* Upper bound of the encoded size of a w x h RGB image, 0 for an unknown format or empty image. */
 size_t artefact_image_encode_bound(uint32_t w, uint32_t h, uint32_t format);
 /* Encodes w*h*3 bytes of row-major interleaved RGB into out (cap bytes, at least the bound)

@@ -1,7 +1,7 @@
 """Image files (csrc/image_encode.cpp), the save step of the reference's CLI
-(backend/artefact-cli/main.rs:156): PNG, BMP and TIFF written here, read back with Pillow
-(libpng, its BMP reader, libtiff) and compared byte for byte with the RGB input; the CLI's
-output path rules (main.rs:57-98)."""
+(backend/artefact-cli/main.rs:156): PNG, BMP, TIFF and lossless WebP written here, read back with
+Pillow (libpng, its BMP reader, libtiff, libwebp) and compared byte for byte with the RGB input;
+the CLI's output path rules (main.rs:57-98)."""
 import ctypes
 import io
 import struct
@@ -24,12 +24,12 @@ def images():
     return out
 
 
-@pytest.mark.parametrize("fmt", ["png", "bmp", "tiff"])
+@pytest.mark.parametrize("fmt", ["png", "bmp", "tiff", "webp"])
 def test_round_trip_through_pillow(fmt):
     for rgb in images():
         data = artefact_amd.encode_image(rgb, fmt)
         with Image.open(io.BytesIO(data)) as im:
-            assert im.format == {"png": "PNG", "bmp": "BMP", "tiff": "TIFF"}[fmt]
+            assert im.format == {"png": "PNG", "bmp": "BMP", "tiff": "TIFF", "webp": "WEBP"}[fmt]
             assert im.mode == "RGB" and im.size == (rgb.shape[1], rgb.shape[0])
             back = np.asarray(im.convert("RGB"))
         assert np.array_equal(back, rgb), (fmt, rgb.shape)
@@ -102,6 +102,47 @@ def test_bmp_layout():
     assert len(data) == 54 + row * 3
 
 
+def webp_images():
+    """Images that take every prefix-code form of the WebP writer: one and two symbols per
+    channel (simple codes), every byte value in a channel (a code-length code of one length
+    and a run of zeros), skewed histograms (long codes), and a size at the format's limit."""
+    rng = np.random.default_rng(11)
+    out = [np.full((3, 4, 3), (7, 200, 7), np.uint8)]
+    two = np.zeros((5, 9, 3), np.uint8)
+    two[::2] = (1, 0, 255)
+    out.append(two)
+    full = np.arange(256 * 3, dtype=np.uint32).reshape(16, 16, 3).astype(np.uint8)
+    out.append(full)
+    # red and blue take every value exactly once (with green 0): all 256 code lengths are 8,
+    # so the code-length sequence is one token repeated
+    flat = np.zeros((16, 16, 3), np.uint8)
+    flat[..., 0] = np.arange(256, dtype=np.uint8).reshape(16, 16)
+    flat[..., 2] = 255 - flat[..., 0]
+    out.append(flat)
+    skew = rng.geometric(0.35, (64, 200, 3)).clip(0, 255).astype(np.uint8)
+    out.append(skew)
+    out.append(rng.integers(0, 256, (2, 16384, 3), dtype=np.uint8))
+    return out
+
+
+def test_webp_is_lossless_vp8l_and_every_code_form_decodes():
+    for rgb in webp_images():
+        data = artefact_amd.encode_image(rgb, "webp")
+        assert data[:4] == b"RIFF" and data[8:16] == b"WEBPVP8L"
+        riff, = struct.unpack("<I", data[4:8])
+        chunk, = struct.unpack("<I", data[16:20])
+        assert riff == len(data) - 8 and chunk + (chunk & 1) == len(data) - 20
+        assert data[20] == 0x2F
+        bits = int.from_bytes(data[21:25], "little")
+        assert (bits & 0x3FFF) + 1 == rgb.shape[1] and ((bits >> 14) & 0x3FFF) + 1 == rgb.shape[0]
+        with Image.open(io.BytesIO(data)) as im:
+            assert im.format == "WEBP" and im.size == (rgb.shape[1], rgb.shape[0])
+            back = np.asarray(im.convert("RGB"))
+        assert np.array_equal(back, rgb), rgb.shape
+    lib = artefact_amd._lib.load()
+    assert lib.artefact_image_encode_bound(16385, 2, artefact_amd._lib.ARTEFACT_IMAGE_WEBP) == 0
+
+
 def test_bound_and_errors():
     lib = artefact_amd._lib.load()
     assert lib.artefact_image_encode_bound(0, 5, 0) == 0
@@ -110,12 +151,12 @@ def test_bound_and_errors():
     small = np.empty(10, np.uint8)
     n = ctypes.c_size_t()
     U8 = artefact_amd._lib.U8_P
-    for fmt in (0, 1, 2):
+    for fmt in (0, 1, 2, 3):
         rc = lib.artefact_image_encode(rgb.ctypes.data_as(U8), 4, 4, fmt, small.ctypes.data_as(U8), small.size,
                                        ctypes.byref(n))
         assert rc == -1 and n.value == 0
     with pytest.raises(artefact_amd.ImageFormatError):
-        artefact_amd.encode_image(rgb, "webp")
+        artefact_amd.encode_image(rgb, "gif")
 
 
 @pytest.mark.parametrize("inp,out,fmt,want", [
@@ -138,7 +179,7 @@ def test_cli_output_path_errors(out, fmt):
 
 def test_save_picks_the_format_from_the_extension(tmp_path):
     rgb = np.stack(synth.synth_rgb(16, 24, 2), -1)
-    for name, kind in (("a.png", "PNG"), ("a.BMP", "BMP"), ("a.tif", "TIFF"), ("a.tiff", "TIFF")):
+    for name, kind in (("a.png", "PNG"), ("a.BMP", "BMP"), ("a.tif", "TIFF"), ("a.tiff", "TIFF"), ("a.webp", "WEBP")):
         p = tmp_path / name
         artefact_amd.save_image(rgb, str(p))
         with Image.open(p) as im:
