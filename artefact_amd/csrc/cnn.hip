@@ -890,16 +890,17 @@ int cnn_predict_planes(artefact_cnn *net, const float *const *planes, const floa
                                net->ws);
             net->ws_stale = false;
         }
-        // Bands of SB columns; row segments sized for about two workgroups per compute unit (a
-        // multiple of 3 rows, at least 12, so that the pipeline fill stays a modest share).
+        // Bands of SB columns; row segments sized for about two workgroups per compute unit over
+        // all the launch's planes (a multiple of 3 rows, at least 12, so that the pipeline fill
+        // stays a modest share).
         const int nbx = ((int)w + SB - 1) / SB;
-        int nby = std::max(1, (2 * std::max(net->cus, 1) + nbx - 1) / nbx);
-        int rh = ((int)h + nby - 1) / nby;
-        rh = std::max(12, (rh + 2) / 3 * 3);
-        nby = ((int)h + rh - 1) / rh;
         for (uint32_t i0 = 0; i0 < nplanes && ok; i0 += SPLANES) {
             StreamPlanes pp{};
             const uint32_t n = std::min<uint32_t>(SPLANES, nplanes - i0);
+            int nby = std::max(1, (2 * std::max(net->cus, 1) + nbx * (int)n - 1) / (nbx * (int)n));
+            int rh = ((int)h + nby - 1) / nby;
+            rh = std::max(12, (rh + 2) / 3 * 3);
+            nby = ((int)h + rh - 1) / rh;
             for (uint32_t k = 0; k < n; k++) {
                 pp.x0[k] = planes[i0 + k];
                 pp.out[k] = out[i0 + k];

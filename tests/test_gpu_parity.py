@@ -317,6 +317,21 @@ def test_projection_kernels_bit_exact(name, iters, env, monkeypatch):
         assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
 
 
+@pytest.mark.parametrize("name", ["tgv_444_72x40", "tgv_420_96x64"])
+def test_two_channel_plan_bit_exact(name):
+    """Two channels (the luma and the first chroma component of a colour workload): the C = 2
+    instances of the gradient sweep, the norm accumulator and the per-channel projection."""
+    wl = synth.workload(name, seed=43)
+    W, H = wl["max_w"], wl["max_h"]
+    coefs = wl["coefs"][:2]
+    print(f"case two channels {name}", flush=True)
+    gpu = artefact_amd.compute(2, coefs, wl["weight"], wl["pweight"], 6, W, H, W * H, mode=wl["mode"])
+    ref = oracle.compute(coefs, wl["weight"], wl["pweight"], 6, W, H, norm_mode="f64")
+    assert len(gpu) == 2
+    for c, (a, b) in enumerate(zip(gpu, ref)):
+        assert bits_equal(a, b), f"channel {c}: max abs diff {np.abs(a - b).max()}"
+
+
 @pytest.mark.parametrize("wgs", ["0", "1", "7", "40"])
 @pytest.mark.parametrize("name", ["tgv_420_208x144", "tgv_444_72x40", "tv_luma_64x64"])
 def test_projection_workgroup_counts_bit_exact(name, wgs, monkeypatch):
