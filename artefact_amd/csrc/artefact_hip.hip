@@ -1315,8 +1315,8 @@ __global__ __launch_bounds__(NT) void k_init_upsample(const ChanDev ch, const fl
 // image: component sample i sits at grid coordinate (i + 0.5) * s - 0.5, and the edges clamp.
 // Without subsampling (s = 1) every weight is 0 or 1 and the plane is copied.  Replication
 // (k_init_upsample) would leave a TV jump every s pixels in a smooth prediction.  One launch
-// for all channels (blockIdx.z); a workgroup covers NT pixels of a row, and the rows stride by
-// gridDim.y.
+// for all channels (blockIdx.z); a thread writes 4 adjacent pixels (one 16-byte store; W is a
+// multiple of 8) of every gridDim.y-th row.
 struct UpsampleArgs {
     const float *plane[3];
     float *out[3];
@@ -1325,16 +1325,23 @@ struct UpsampleArgs {
 };
 __global__ __launch_bounds__(NT) void k_upsample_linear(const UpsampleArgs u)
 {
-    const uint32_t c = blockIdx.z, xx = blockIdx.x * NT + threadIdx.x;
-    if (xx >= u.W) return;
+    const uint32_t c = blockIdx.z, xb = (blockIdx.x * NT + threadIdx.x) * 4;
+    if (xb >= u.W) return;
     const uint32_t rw = u.rw[c], rh = u.rh[c];
     auto clamp = [](float t, uint32_t n) {
         const int k = (int)t;
         return (uint32_t)(k < 0 ? 0 : (k > (int)n - 1 ? (int)n - 1 : k));
     };
-    const float uu = ((float)xx + 0.5f) / (float)u.hs[c] - 0.5f;
-    const float fu = floorf(uu), au = uu - fu;
-    const uint32_t x0 = clamp(fu, rw), x1 = clamp(fu + 1.0f, rw);
+    uint32_t x0[4], x1[4];
+    float au[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float uu = ((float)(xb + i) + 0.5f) / (float)u.hs[c] - 0.5f;
+        const float fu = floorf(uu);
+        au[i] = uu - fu;
+        x0[i] = clamp(fu, rw);
+        x1[i] = clamp(fu + 1.0f, rw);
+    }
     const float *plane = u.plane[c];
     float *out = u.out[c];
     for (uint32_t y = blockIdx.y; y < u.H; y += gridDim.y) {
@@ -1342,9 +1349,14 @@ __global__ __launch_bounds__(NT) void k_upsample_linear(const UpsampleArgs u)
         const float fv = floorf(v), av = v - fv;
         const uint32_t y0 = clamp(fv, rh), y1 = clamp(fv + 1.0f, rh);
         const float *r0 = plane + (size_t)y0 * rw, *r1 = plane + (size_t)y1 * rw;
-        const float top = au == 0.0f ? r0[x0] : (1.0f - au) * r0[x0] + au * r0[x1];
-        const float bot = au == 0.0f ? r1[x0] : (1.0f - au) * r1[x0] + au * r1[x1];
-        out[(size_t)y * u.W + xx] = av == 0.0f ? top : (1.0f - av) * top + av * bot;
+        float o[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float top = au[i] == 0.0f ? r0[x0[i]] : (1.0f - au[i]) * r0[x0[i]] + au[i] * r0[x1[i]];
+            const float bot = au[i] == 0.0f ? r1[x0[i]] : (1.0f - au[i]) * r1[x0[i]] + au[i] * r1[x1[i]];
+            o[i] = av == 0.0f ? top : (1.0f - av) * top + av * bot;
+        }
+        *reinterpret_cast<float4 *>(out + (size_t)y * u.W + xb) = make_float4(o[0], o[1], o[2], o[3]);
     }
 }
 
@@ -3207,7 +3219,7 @@ int artefact_hip_plan_warm_init(artefact_hip_plan *pl, artefact_cnn *net)
     }
     ua.W = pl->W;
     ua.H = pl->H;
-    hipLaunchKernelGGL(k_upsample_linear, dim3((pl->W + NT - 1) / NT, std::min<uint32_t>(pl->H, 65535u), pl->C),
+    hipLaunchKernelGGL(k_upsample_linear, dim3((pl->W / 4 + NT - 1) / NT, std::min<uint32_t>(pl->H, 128u), pl->C),
                        dim3(NT), 0, pl->stream, ua);
     // Projection of the prediction: the projection kernel with zero norms (the accumulator
     // launch_init cleared: no descent step) reads y = prediction and writes the projected
