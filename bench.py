@@ -389,11 +389,55 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
     planes[f"plain_{K}"] = cold.download()
     cold.close()
     f_plain, f_warm = res["plain_100_objective"], res[f"cnn_plus_{K}_objective"]
+    res["time_to_plain_100_objective"] = warm_time_to_objective(net, coefs, W, H, f_plain, res["plain_100_ms"],
+                                                                 timed, warm_start)
     res["warm_time_fraction_of_plain"] = round(res[f"cnn_plus_{K}_ms"] / res["plain_100_ms"], 4)
     res["warm_objective_rel_to_plain"] = float(f"{(f_warm - f_plain) / f_plain:.4g}")
     res["reaches_plain_objective_within_0.1pct"] = bool(f_warm <= f_plain * 1.001)
     res["used_fallback"] = bool(f_warm > f_plain)
     return res, planes
+
+
+def warm_time_to_objective(net, coefs, W, H, target, plain_ms, timed, warm_start, schedules=(250, 400)):
+    """The fewest plain iterations K after the CNN start whose objective reaches target (plain
+    100's), for refinement steps of an S-iteration schedule: bisection over K in 15..100 for each
+    S (the objective falls with K), then the fastest (K, S) timed like the other solves."""
+    import artefact_amd
+
+    pw = [0.001] * 3
+
+    def objective(K, S):
+        with artefact_amd.Plan(coefs, 0.3, pw, K, W, H) as p:
+            artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_iterations(p._h, S), "step")
+            warm_start(p)
+            p.iterate()
+            return p.objective()[0]
+
+    best = None
+    for S in schedules:
+        lo, hi = 15, 100
+        if objective(hi, S) > target:
+            continue
+        while hi - lo > 1:
+            mid = (lo + hi) // 2
+            if objective(mid, S) <= target:
+                hi = mid
+            else:
+                lo = mid
+        if best is None or hi < best[0]:
+            best = (hi, S)
+    if best is None:
+        return {"reached": False, "searched": "K in 15..100, steps of the %s-iteration schedules" % list(schedules)}
+    K, S = best
+    with artefact_amd.Plan(coefs, 0.3, pw, K, W, H) as p:
+        artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_iterations(p._h, S), "step")
+        ms = timed(p, warm_start)
+        warm_start(p)
+        p.iterate()
+        f = p.objective()[0]
+    return {"reached": True, "cnn_plus_K": K, "step_iterations": S, "ms": round(ms, 4),
+            "objective_rel_to_plain": float(f"{(f - target) / target:.4g}"),
+            "fraction_of_plain_time": round(ms / plain_ms, 4)}
 
 
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -651,6 +695,7 @@ def bench_warmstart(args) -> int:
             pl.close()
             sweep[f"{kind}_{k}"] = {"ms": round(t_k, 4), "objective_rel_to_plain_100": float(f"{(f_k - f_plain) / f_plain:.4g}")}
     fallback = f_warm > f_plain
+    reach = warm_time_to_objective(net, coefs, W, H, f_plain, t_plain, timed, warm_start)
     print(f"bench: plain {t_plain:.3f} ms F={f_plain:.6g}; warm {t_warm:.3f} ms F={f_warm:.6g}; "
           f"plain {K} {t_cold:.3f} ms F={f_cold:.6g}", file=sys.stderr, flush=True)
     psnr = None
@@ -688,6 +733,7 @@ def bench_warmstart(args) -> int:
             f"plain_{K}_ms": round(t_cold, 4), f"plain_{K}_objective": f_cold,
             "psnr_vs_fp64_converged_db": psnr,
             "iteration_sweep": sweep,
+            "time_to_plain_100_objective": reach,
             "training": {k: v.item() if hasattr(v, "item") and np.asarray(v).ndim == 0 else None
                          for k, v in meta.items() if k != "params"},
         },

@@ -17,7 +17,8 @@ subsample of its planes).
 * The 64-image batch (sides 256-2048, 4:2:0 / 4:4:4, q30-90, TV / TGV): every image bit-identical
   to the oracle after 3 iterations (graph of all images, autotuned), and every block of every
   image re-quantising after the full 100.
-* The learned warm start at 2048x1536 4:2:0 q50 (CNN + 15 iterations against plain 100).
+* The learned warm start at 2048x1536 4:2:0 q50 (CNN + 15 iterations against plain 100, and
+  CNN + 70 iterations reaching plain 100's objective).
 """
 import json
 import os
@@ -202,3 +203,23 @@ def test_warm_start_full_size():
           f"plain 100 {p_plain:.2f} dB", flush=True)
     assert p_warm > p_dec + 3.0
     assert p_plain > p_warm
+
+
+def test_warm_start_reaches_plain_objective():
+    """2048x1536 4:2:0 q50: with 70 iterations after the CNN start (the step of the 250-iteration
+    schedule), the warm start reaches plain 100's objective without the fallback, in 70 % of the
+    plain loop's iterations, and every block still re-quantises."""
+    coefs = synth.encode(synth.synth_rgb(1536, 2048, 0), subsampling="420", quality=50)
+    W, H = coefs[0].rounded_px_w, coefs[0].rounded_px_h
+    print("case warm start 2048x1536 q50, K=70, step of 250", flush=True)
+    _, f_plain = solve({"coefs": coefs, "weight": 0.3, "pweight": [0.001] * 3, "iterations": 100, "max_w": W,
+                        "max_h": H, "mode": "tgv"})
+    net = warmstart.WarmStartNet.load()
+    res = warmstart.warm_solve(coefs, W, H, net, iterations=70, plain_objective=f_plain, plain_iterations=100,
+                               step_iterations=250)
+    net.close()
+    print(f"warm objective {res.warm_objective:.7g} vs plain-100 {f_plain:.7g} "
+          f"({(res.warm_objective - f_plain) / f_plain:+.3e})", flush=True)
+    assert not res.used_fallback
+    assert res.objective <= f_plain
+    assert requant_deviation(res.planes, coefs, W, H) <= 0.5 + 1e-3
