@@ -377,6 +377,10 @@ template <int C, bool TGV, bool FAST = false, bool WIDE = false>
 __global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
 {
     __shared__ DD lred[WAVES][3][64];
+    // Rows shared by the strips of consecutive waves: the 4 rows around each of the workgroup's
+    // 3 inner strip boundaries, x and x_prev per channel, handed from the lower strip's wave
+    // (its first rows) to the upper one's (its last rows).
+    __shared__ float hal[WAVES - 1][4][2 * C][64];
 
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -440,7 +444,19 @@ __global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
         else
             vpr[c] = (out_lane && cx < crw[c]) ? poff[c] + (uint32_t)cx * 4u : a.prob_bytes;
     }
+    // Halo sharing needs the hand-over (steps 0-3 of the lower wave, then a barrier) to come
+    // before the first row the upper wave takes (step RW - 1).
+    const bool share = RW >= 5;
     auto rowload = [&](int t) {
+        const int kk = t - (r0 - 2);  // this strip's row index, 0 .. RW + 3
+        if (share && wave < WAVES - 1 && kk >= RW) {
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                nx[c] = hal[wave][kk - RW][2 * c][lane];
+                nxp[c] = hal[wave][kk - RW][2 * c + 1][lane];
+            }
+            return;
+        }
         const uint32_t tr = (uint32_t)min(max(t, 0), H - 1);
 #pragma unroll
         for (int c = 0; c < C; c++) {
@@ -499,6 +515,15 @@ __global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
         constexpr int k1 = (km + 2) % 3, k2 = (km + 1) % 3;  // rings: one and two rows back
         const int t = r0 - 2 + k;
         float *yt = yring[km], *dxt = dxring[km];
+        if (k < 4 && share && wave > 0) {
+            // The lower wave's first rows are the upper wave's last: hand them over (here, where
+            // the loads have arrived anyway).
+#pragma unroll
+            for (int c = 0; c < C; c++) {
+                hal[wave - 1][k][2 * c][lane] = nx[c];
+                hal[wave - 1][k][2 * c + 1][lane] = nxp[c];
+            }
+        }
 #pragma unroll
         for (int c = 0; c < C; c++) yt[c] = fista(nx[c], nxp[c], f);
         {
@@ -695,6 +720,7 @@ __global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
     step(1, IC<1>{}, IC<1>{});
     step(2, IC<2>{}, IC<2>{});
     step(3, IC<0>{}, IC<3>{});
+    if (share) __syncthreads();  // the shared rows are in LDS before any upper wave takes them
     // Whole groups of three steps (no exit inside a group, so values that one step hands to the
     // next stay in one basic block), then the remaining 0-2 steps.
     int k = 4;
