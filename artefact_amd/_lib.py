@@ -88,6 +88,7 @@ class artefact_tuning(ctypes.Structure):
         ("project_kernel", ctypes.c_uint32),
         ("ms_per_iteration", ctypes.c_float),
         ("project_workgroups", ctypes.c_uint32),
+        ("xcd_bands", ctypes.c_uint32),
     ]
 
 

@@ -150,7 +150,8 @@ class Plan:
         t = _lib.artefact_tuning()
         _lib.check(self._lib.artefact_hip_plan_tune(self._h, ctypes.byref(t)), "artefact_hip_plan_tune")
         return {"kg_rows": t.kg_rows, "project_kernel": {0: "per-channel", 1: "fused-lds", 2: "fused-lane"}[t.project_kernel],
-                "ms_per_iteration": t.ms_per_iteration, "project_workgroups": t.project_workgroups}
+                "ms_per_iteration": t.ms_per_iteration, "project_workgroups": t.project_workgroups,
+                "xcd_bands": bool(t.xcd_bands)}
 
     def download_rgb(self, real_w: int, real_h: int) -> np.ndarray:
         """Output stage on the GPU (lib.rs:159-211): uint8 (real_h, real_w, 3)."""

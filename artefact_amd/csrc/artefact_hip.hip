@@ -270,6 +270,7 @@ struct IterArgs {
     double *rnorm;           // [3] 1/norm, correctly rounded f64
     double *norm_sq;         // [3]
     uint32_t kg_rows;        // k_gradient output rows per wave
+    uint32_t xcd_bands;      // projection workgroups on the XCD that swept their rows (see k_project_lane)
 };
 
 // ---- Norm accumulator ----
@@ -964,7 +965,9 @@ __global__ __launch_bounds__(NT) void k_project(const IterArgs a)
     __shared__ ProjShared sh;
     __shared__ NormShared ns;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    unsigned int tx = blockIdx.x, ty = blockIdx.y;
+    if (a.xcd_bands) xcd_tile(tx, ty);  // the XCD's band of rows, as in k_gradient
+    const int x0 = (int)tx * TW, y0 = (int)ty * TH;
     const int W = a.W, H = a.H;
     const int tr = tid >> 4, tc = (tid & 15) * 4;
     const int qy = y0 + tr, qx = x0 + tc;
@@ -1125,7 +1128,9 @@ __global__ __launch_bounds__(NT) void k_project_fused(const IterArgs a)
     __shared__ FusedShared<C, M0, M1, M2> sh;
     __shared__ NormShared ns;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    unsigned int tx = blockIdx.x, ty = blockIdx.y;
+    if (a.xcd_bands) xcd_tile(tx, ty);  // the XCD's band of rows, as in k_gradient
+    const int x0 = (int)tx * TW, y0 = (int)ty * TH;
     const int W = a.W, H = a.H;
     const int tr = tid >> 4, tc = (tid & 15) * 4;
     const int qy = y0 + tr, qx = x0 + tc;
@@ -1769,7 +1774,9 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
     // The wave's four 8x8 blocks sit side by side in one 8-row band: super-tile blocks (0,0),
     // (0,1), (1,0), (1,1) are image blocks 0, 1, 2, 3 from the left; a workgroup covers 2 x 2
     // such bands of 32 columns (its TW x TH tile).
-    const int x0 = blockIdx.x * TW + (wave & 1) * 32, y0 = blockIdx.y * TH + (wave >> 1) * 8;
+    unsigned int tx = blockIdx.x, ty = blockIdx.y;
+    if (a.xcd_bands) xcd_tile(tx, ty);  // the XCD's band of rows, as in k_gradient
+    const int x0 = (int)tx * TW + (wave & 1) * 32, y0 = (int)ty * TH + (wave >> 1) * 8;
     // X enters in the A operand layout (lane l: row l % 16, columns 4 (l / 16) .. + 3), so each
     // lane loads and stores one float4 per plane.  B operands: lane l holds element
     // (k = 4 (l / 16) + t, n = l % 16).  Forward stages use Db^T, inverse stages Db.
@@ -2118,17 +2125,53 @@ constexpr int lane_blocks_per_wave() { return MODE == PM_SUB22 ? 4 : 8; }
 
 // One wave per unit = 8 (full resolution) or 4 (4:2:0 chroma) consecutive blocks of a block row;
 // units of channel 0 first.  u0, u1, u2: cumulative unit counts after channels 0, 1, 2.  Each
-// workgroup forms the norms once and its waves then take units gridDim.x * NT / 64 apart, so a
-// grid smaller than the unit count spreads that cost over several units per wave.
+// workgroup forms the norms once and its waves then take several units, so a grid smaller than
+// the unit count spreads that cost.
+//
+// XCD bands (a.xcd_bands).  The workgroups an XCD runs (workgroup i runs on XCD i % NXCD) take
+// the units of the block rows that start in its eighth of the grid's rows, channel after
+// channel: the rows k_gradient's workgroups on the same XCD sweep (xcd_tile), so the y and g
+// this kernel reads and the x_new and prob it writes for the next sweep stay in that XCD's L2,
+// which keeps a kernel's lines for the next one (measured: a 32 MiB buffer read back in 3.4 us
+// on the XCDs that wrote it, 18.5 us on the others).  That pays where an iteration's planes fit
+// the L2s (512^2 luma: 12.1 -> 10.4 us per iteration); on large grids the bands cost 1-2 %, so
+// the plan decides (set_layout, the autotuner).
 template <int C, int M0, int M1, int M2>
 __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t u0, uint32_t u1, uint32_t u2)
 {
     __shared__ NormShared ns;
     const int lane = threadIdx.x & 63;
     projection_norms<C>(a, ns);  // before any wave leaves: it has barriers
-    const uint32_t stride = gridDim.x * (NT / 64);
-    for (uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (NT / 64) + (threadIdx.x >> 6)); unit < u2;
-         unit += stride) {
+    // A grid of fewer than NXCD workgroups leaves XCDs without one: then every workgroup takes
+    // from the whole grid.
+    const bool bands = a.xcd_bands && gridDim.x >= NXCD;
+    // The units this workgroup's waves take, as an index k into a list: unit k itself, or with
+    // bands the k-th unit of this XCD's band (its block rows of channel 0, then 1, then 2).
+    uint32_t first[3] = {0, u0, u1}, count[3] = {u0, u1 - u0, u2 - u1};
+    uint32_t wg = blockIdx.x, nwg = gridDim.x;
+    if (bands) {
+        const uint32_t xcd = blockIdx.x % NXCD;
+        wg = blockIdx.x / NXCD;
+        nwg = (gridDim.x - xcd + NXCD - 1) / NXCD;
+        const uint32_t r_lo = xcd * a.H / NXCD, r_hi = (xcd + 1) * a.H / NXCD;
+        auto band = [&](auto MODE_C, auto C_C) {
+            constexpr int MODE = decltype(MODE_C)::value, c = decltype(C_C)::value;
+            constexpr uint32_t BPW = lane_blocks_per_wave<MODE>(), ROWS = MODE == PM_SUB22 ? 16 : 8;  // grid rows per block row
+            const ChanDev &ch = a.ch[c];
+            const uint32_t per_row = (ch.bw + BPW - 1) / BPW;
+            const uint32_t by_lo = min((r_lo + ROWS - 1) / ROWS, ch.bh), by_hi = min((r_hi + ROWS - 1) / ROWS, ch.bh);
+            first[c] += by_lo * per_row;
+            count[c] = (by_hi - by_lo) * per_row;
+        };
+        band(IC<M0>{}, IC<0>{});
+        if constexpr (C > 1) band(IC<M1>{}, IC<1>{});
+        if constexpr (C > 2) band(IC<M2>{}, IC<2>{});
+    }
+    const uint32_t n = count[0] + (C > 1 ? count[1] : 0) + (C > 2 ? count[2] : 0), stride = nwg * (NT / 64);
+    for (uint32_t k = __builtin_amdgcn_readfirstlane(wg * (NT / 64) + (threadIdx.x >> 6)); k < n; k += stride) {
+        const uint32_t unit = k < count[0] ? first[0] + k
+                              : k < count[0] + count[1] ? first[1] + (k - count[0])
+                                                        : first[2] + (k - count[0] - count[1]);
         auto run = [&](auto MODE_C, auto C_C, uint32_t local) {
             constexpr int MODE = decltype(MODE_C)::value, c = decltype(C_C)::value;
             constexpr int BPW = lane_blocks_per_wave<MODE>();
@@ -2163,6 +2206,7 @@ struct artefact_hip_plan {
     int lane_project = 0;  // fused layouts: 1 = k_project_lane (no LDS), 0 = k_project_fused
     uint32_t lane_units[3] = {0, 0, 0};  // k_project_lane: cumulative units (one wave's blocks) per channel
     uint32_t lane_wgs = 0;               // k_project_lane: workgroup cap (0: one unit per wave)
+    bool xcd_bands = false;              // projection workgroups by XCD row band (IterArgs::xcd_bands)
     uint32_t precision = ARTEFACT_PRECISION_F32;  // ARTEFACT_PRECISION_BF16X3: MFMA projection
     float qtab[3][64] = {};                       // quantisation tables per channel, f32 (warm-start features)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
@@ -2267,6 +2311,7 @@ IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool writ
     a.rnorm = pl->rnorm;
     a.norm_sq = pl->norm_sq;
     a.kg_rows = (uint32_t)pl->rows;
+    a.xcd_bands = pl->xcd_bands ? 1u : 0u;
     return a;
 }
 
@@ -2540,6 +2585,10 @@ void set_layout(artefact_hip_plan *pl)
         cus = 256;
     pl->lane_wgs = 3u * (uint32_t)cus;
     if (const char *e = getenv("ARTEFACT_LANE_WGS")) pl->lane_wgs = (uint32_t)atoi(e);
+    // Projection workgroups by XCD row band where the iteration's planes (x, x_prev, y, g: 16 C N
+    // bytes) fit the 8 XCDs' L2s (4 MiB each); the autotuner times the other choice too.
+    pl->xcd_bands = 16.0 * (double)pl->C * (double)pl->N <= 32.0 * 1048576.0;
+    if (const char *e = getenv("ARTEFACT_XCD_BANDS")) pl->xcd_bands = atoi(e) != 0;
     if (const char *e = getenv("ARTEFACT_GENERIC_PROJECT"))
         if (atoi(e)) pl->layout = LAYOUT_GENERIC;
     // Lets callers of the one-shot entry points (artefact_hip_compute / _process, batches) opt
@@ -2571,6 +2620,7 @@ struct TuneChoice {
     int rows, layout, lane;
     uint32_t wgs;  // k_project_lane's workgroup cap
     float ms;
+    bool bands;    // projection workgroups by XCD row band
 };
 std::mutex g_tune_mu;
 std::map<std::string, TuneChoice> g_tune_cache;
@@ -2646,7 +2696,7 @@ void drop_graph(artefact_hip_plan *pl)
 int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
 {
     const std::string key = tune_key(pl);
-    TuneChoice best{pl->rows, pl->layout, pl->lane_project, pl->lane_wgs, -1.0f};
+    TuneChoice best{pl->rows, pl->layout, pl->lane_project, pl->lane_wgs, -1.0f, pl->xcd_bands};
     {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune_cache.find(key);
@@ -2658,6 +2708,7 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
             pl->layout = c.layout;
             pl->lane_project = c.lane;
             pl->lane_wgs = c.wgs;
+            pl->xcd_bands = c.bands;
             set_rows(pl, c.rows);
         };
         const uint32_t nt = std::min<uint32_t>(pl->iterations, 16);
@@ -2700,7 +2751,7 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
                 set_rows(pl, r);
                 const float t = time_loop(pl, nt);
                 if (t < 0.0f) return ARTEFACT_E_HIP;
-                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, pj.first, pj.second, pl->lane_wgs, t};
+                if (best.ms < 0.0f || t < best.ms) best = TuneChoice{r, pj.first, pj.second, pl->lane_wgs, t, pl->xcd_bands};
             }
         // The lane kernel's workgroup count on the winner: 2-4 per CU, or one unit per wave.
         if (best.lane && best.layout != LAYOUT_GENERIC && !getenv("ARTEFACT_LANE_WGS")) {
@@ -2714,12 +2765,25 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
                 pl->lane_wgs = w;
                 const float t = time_loop(pl, nt);
                 if (t < 0.0f) return ARTEFACT_E_HIP;
-                if (t < best.ms) best = TuneChoice{best.rows, best.layout, best.lane, w, t};
+                if (t < best.ms) best = TuneChoice{best.rows, best.layout, best.lane, w, t, best.bands};
+            }
+        }
+        // XCD bands on the winner: the other choice.
+        if (!getenv("ARTEFACT_XCD_BANDS")) {
+            TuneChoice other = best;
+            other.bands = !best.bands;
+            apply(other);
+            const float t = time_loop(pl, nt);
+            if (t < 0.0f) return ARTEFACT_E_HIP;
+            if (t < best.ms) {
+                best = other;
+                best.ms = t;
             }
         }
         // Guard against a noisy pick: time the winner and the default alternately, twice each,
         // and keep the winner only if it is faster both times.
-        if (best.rows != dflt.rows || best.layout != dflt.layout || best.lane != dflt.lane || best.wgs != dflt.wgs) {
+        if (best.rows != dflt.rows || best.layout != dflt.layout || best.lane != dflt.lane || best.wgs != dflt.wgs ||
+            best.bands != dflt.bands) {
             int wins = 0;
             float tb = 0.0f, td = 0.0f;
             for (int round = 0; round < 2; round++) {
@@ -2741,6 +2805,7 @@ int tune_plan(artefact_hip_plan *pl, TuneChoice *out)
     pl->layout = best.layout;
     pl->lane_project = best.lane;
     pl->lane_wgs = best.wgs;
+    pl->xcd_bands = best.bands;
     set_rows(pl, best.rows);
     drop_graph(pl);
     pl->initialized = false;
@@ -3324,6 +3389,7 @@ int artefact_hip_plan_tune(artefact_hip_plan *pl, artefact_tuning *out)
         out->project_kernel = t.layout == LAYOUT_GENERIC ? 0u : (t.lane ? 2u : 1u);
         out->ms_per_iteration = t.ms;
         out->project_workgroups = t.wgs;
+        out->xcd_bands = t.bands ? 1u : 0u;
     }
     return ARTEFACT_OK;
 }

@@ -149,13 +149,14 @@ typedef struct {
     uint32_t project_kernel;
     float ms_per_iteration;
     uint32_t project_workgroups;
+    uint32_t xcd_bands;  /* 1: projection workgroups on the XCD whose k_gradient workgroups swept their rows */
 } artefact_tuning;
 
 /* Times candidate strip heights, projection kernels and projection workgroup counts on the
  * plan (a few iterations each) and keeps the fastest; the choice is cached per geometry for the
  * process.  Leaves the plan uninitialised (call artefact_hip_plan_init next).  ARTEFACT_KG_ROWS,
- * ARTEFACT_GENERIC_PROJECT, ARTEFACT_PROJECT_KERNEL and ARTEFACT_LANE_WGS in the environment pin
- * their parameter.  No reference counterpart: the reference has one fixed schedule. */
+ * ARTEFACT_GENERIC_PROJECT, ARTEFACT_PROJECT_KERNEL, ARTEFACT_LANE_WGS and ARTEFACT_XCD_BANDS in
+ * the environment pin their parameter.  No reference counterpart: the reference has one fixed schedule. */
 int artefact_hip_plan_tune(artefact_hip_plan *plan, artefact_tuning *out);
 
 /* Many images in one hipGraph (the batched form).  Each image is one compute() call's
