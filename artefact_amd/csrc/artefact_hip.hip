@@ -2633,6 +2633,10 @@ std::string tune_key(const artefact_hip_plan *pl)
     for (uint32_t c = 0; c < pl->C; c++)
         k += " " + std::to_string(pl->ch[c].rw) + "x" + std::to_string(pl->ch[c].rh) + "/" +
              std::to_string(pl->ch[c].hs) + std::to_string(pl->ch[c].vs);
+    // A choice made under environment pins is not one to reuse without them, nor the reverse.
+    for (const char *pin : {"ARTEFACT_KG_ROWS", "ARTEFACT_LANE_WGS", "ARTEFACT_PROJECT_KERNEL",
+                            "ARTEFACT_GENERIC_PROJECT", "ARTEFACT_XCD_BANDS"})
+        if (const char *v = getenv(pin)) k += std::string(" ") + pin + "=" + v;
     return k;
 }
 
