@@ -249,7 +249,7 @@ def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective:
     The prediction is not bit-exact to anything (the CNN runs in bf16), so the refinement takes
     the fast f32 gradient sweep by default (``precision``); the plain result is the exact loop.
     Two settings measured at 2048x1536 4:2:0 q50 against plain 100 (6.4 ms): 15 iterations with
-    the step of 100 take 23 % of its time and end 6.1 % above its objective (the fallback is
+    the step of 100 take 22 % of its time and end 6.0 % above its objective (the fallback is
     taken); 63 iterations with the step of 250 reach its objective in 64 % of its time."""
     with Plan(coefs, weight, list(pweight), iterations, W, H) as plan:
         plan.set_precision(precision)
@@ -282,9 +282,9 @@ def main(argv=None) -> int:
     import argparse
 
     ap = argparse.ArgumentParser(description="train the warm-start CNN on solver-generated pairs")
-    ap.add_argument("--crops", type=int, default=4000, help="crops of 2 size x 2 size (six samples each)")
+    ap.add_argument("--crops", type=int, default=8000, help="crops of 2 size x 2 size (six samples each)")
     ap.add_argument("--size", type=int, default=256)
-    ap.add_argument("--steps", type=int, default=30000)
+    ap.add_argument("--steps", type=int, default=60000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--target-iterations", type=int, default=300)
