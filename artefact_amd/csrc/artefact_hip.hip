@@ -311,32 +311,38 @@ __device__ __forceinline__ void acc_add(long long *acc, double x, int j)
     __hip_atomic_fetch_add(acc + d, (long long)(bits >> 63) ? -v : v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// The accumulated sum of one channel (the ACC_SLOTS copies already added, in `dig`), rounded
-// once to f64 (to nearest, ties to even); NaN if a value was out of range.
-__device__ __forceinline__ double acc_value(const long long *dig)
+// The accumulated sum of one channel, rounded once to f64 (to nearest, ties to even); NaN if a
+// value was out of range.  Called by a whole wave: lane d < ACC_DIGITS holds digit d with the
+// ACC_SLOTS copies added, lane ACC_DIGITS the flag, the other lanes 0.  The digits' carries are
+// resolved in parallel: each round every digit keeps its low 32 bits and hands the signed rest
+// to the digit above.  The first round leaves carries of -1, 0 or +1 only, and such a carry
+// moves up one digit per round, so at most ACC_DIGITS + 1 rounds have any.  Every lane
+// returns the same value.
+__device__ __forceinline__ double acc_value(long long v, int lane)
 {
-    if (dig[ACC_DIGITS] != 0) return __builtin_nan("");
-    unsigned int d32[ACC_DIGITS];
-    long long carry = 0;
-#pragma unroll
-    for (int i = 0; i < ACC_DIGITS; i++) {
-        const long long v = dig[i] + carry;
-        d32[i] = (unsigned int)(v & 0xffffffffll);
-        carry = v >> 32;  // arithmetic shift: a negative running total borrows
+    const bool bad = __ballot(lane == ACC_DIGITS && v != 0) != 0;
+    long long x = lane < ACC_DIGITS ? v : 0;
+    long long top = 0;  // carries out of the top digit, summed
+#pragma unroll 1
+    for (int r = 0; r <= ACC_DIGITS + 1; r++) {
+        const long long carry = x >> 32;  // arithmetic shift: a negative digit borrows
+        if (__ballot(carry != 0) == 0) break;
+        top += __shfl(carry, ACC_DIGITS - 1, 64);
+        const long long cin = __shfl_up(carry, 1, 64);
+        x = lane < ACC_DIGITS ? (x & 0xffffffffll) + (lane > 0 ? cin : 0) : 0;
     }
-    if (carry != 0) return __builtin_nan("");  // a negative total (impossible for sums of squares)
-    int k = -1;
-#pragma unroll
-    for (int i = 0; i < ACC_DIGITS; i++)
-        if (d32[i]) k = i;
-    if (k < 0) return 0.0;
+    // A negative total (impossible for sums of squares) leaves a borrow out of the top digit.
+    if (bad || top != 0) return __builtin_nan("");
+    const unsigned int d32 = (unsigned int)x;
+    const unsigned long long nz = __ballot(d32 != 0);
+    if (nz == 0) return 0.0;
+    const int k = 63 - __builtin_clzll(nz);
     // The top three digits as a 96-bit integer T:R, the digits below as a sticky bit.
-    unsigned long long T = ((unsigned long long)d32[k] << 32) | (k >= 1 ? d32[k - 1] : 0u);
-    unsigned int R = k >= 2 ? d32[k - 2] : 0u;
-    bool sticky = false;
-#pragma unroll
-    for (int i = 0; i < ACC_DIGITS; i++)
-        if (i < k - 2 && d32[i]) sticky = true;
+    const unsigned int dk = (unsigned int)__builtin_amdgcn_readlane((int)d32, k);
+    const unsigned int dk1 = k >= 1 ? (unsigned int)__builtin_amdgcn_readlane((int)d32, k - 1) : 0u;
+    unsigned int R = k >= 2 ? (unsigned int)__builtin_amdgcn_readlane((int)d32, k - 2) : 0u;
+    const bool sticky = k >= 3 && (nz & ((1ull << (k - 2)) - 1)) != 0;
+    unsigned long long T = ((unsigned long long)dk << 32) | dk1;
     const int lz = __builtin_clzll(T);  // < 32: the top digit is non-zero
     if (lz) {
         T = (T << lz) | ((unsigned long long)R >> (32 - lz));
@@ -757,40 +763,37 @@ __global__ __launch_bounds__(NT, 5) void k_gradient(const IterArgs a)
 
 // Gradient norm per channel (compute_step.rs:54-58) from the accumulator k_gradient filled,
 // so every workgroup that forms it gets the same bits, whatever order k_gradient's workgroups
-// ran in.  Called by all threads of a workgroup (it has barriers); on return out.norm[c] and
+// ran in.  Called by all threads of a workgroup (it has a barrier); on return out.norm[c] and
 // out.rnorm[c] hold the results in LDS.  If `writer` is set, the workgroup also stores them,
 // with the squared norm, to the plan's buffers, and clears the next iteration's accumulator
 // (which no kernel of this iteration touches).  k_gradient adds with device-scope atomics;
 // the reads and the clearing are plain accesses in the kernels before and after it, ordered
 // by the kernel boundaries like every other buffer the kernels hand on.
 struct NormShared {
-    long long dig[3][ACC_CH];
     float norm[3];
     double rnorm[3];
 };
 template <int C>
 __device__ __forceinline__ void accumulated_norms(const IterArgs &a, bool writer, NormShared &out)
 {
-    const int tid = threadIdx.x;
-    if (tid < C * ACC_CH) {
-        const int c = tid / ACC_CH, d = tid % ACC_CH;
+    const int tid = threadIdx.x, lane = tid & 63, c = tid >> 6;  // wave c forms channel c
+    if (c < C) {
         long long v = 0;
+        if (lane < ACC_CH) {
 #pragma unroll
-        for (int sl = 0; sl < ACC_SLOTS; sl++) v += a.acc[(size_t)sl * 3 * ACC_CH + c * ACC_CH + d];
-        out.dig[c][d] = v;
-    }
-    __syncthreads();
-    if (tid < C) {
-        const int c = tid;
-        const double sum = acc_value(out.dig[c]);
-        const float norm = (float)sqrt(sum);
-        const double rn = norm != 0.0f ? 1.0 / (double)norm : 0.0;
-        out.norm[c] = norm;
-        out.rnorm[c] = rn;
-        if (writer) {
-            a.norm_sq[c] = sum;
-            a.norms[c] = norm;
-            a.rnorm[c] = rn;
+            for (int sl = 0; sl < ACC_SLOTS; sl++) v += a.acc[(size_t)sl * 3 * ACC_CH + c * ACC_CH + lane];
+        }
+        const double sum = acc_value(v, lane);
+        if (lane == 0) {
+            const float norm = (float)sqrt(sum);
+            const double rn = norm != 0.0f ? 1.0 / (double)norm : 0.0;
+            out.norm[c] = norm;
+            out.rnorm[c] = rn;
+            if (writer) {
+                a.norm_sq[c] = sum;
+                a.norms[c] = norm;
+                a.rnorm[c] = rn;
+            }
         }
     }
     if (writer)
@@ -2141,7 +2144,7 @@ __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t 
 {
     __shared__ NormShared ns;
     const int lane = threadIdx.x & 63;
-    projection_norms<C>(a, ns);  // before any wave leaves: it has barriers
+    projection_norms<C>(a, ns);  // before any wave leaves: it has a barrier
     // A grid of fewer than NXCD workgroups leaves XCDs without one: then every workgroup takes
     // from the whole grid.
     const bool bands = a.xcd_bands && gridDim.x >= NXCD;
