@@ -1888,25 +1888,21 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
 
 // ---------------------------------------------------------------------------------------
 // k_project_lane: the same projection as k_project_fused (same Ooura passes on the same
-// vectors, so the same bits) without LDS or barriers.  Eight lanes own one 8x8 block, one
-// block row each, so rows load and store coalesced; the column passes run on columns obtained
-// by an 8x8 transpose across those lanes.  The block-row index j sits in lane bits 3-5, which
-// makes two of the transpose's three exchange stages plain register swaps between lane halves
+// vectors, so the same bits) without workgroup barriers past the norms.  Eight lanes own one
+// 8x8 block, one block row each; the column passes run on columns obtained by an 8x8 transpose
+// across those lanes.  The block-row index j sits in lane bits 3-5, which makes two of the
+// transpose's three exchange stages plain register swaps between lane halves
 // (v_permlane32_swap, j bit 2) and between 16-lane rows (v_permlane16_swap, j bit 1); only the
 // j bit 0 stage (lane ^ 8, DPP row_ror:8) selects.  A wave takes 8 consecutive blocks of one
-// block row of a full-resolution channel (lane = 8 j + block), or 4 blocks of 4:2:0 chroma
-// (lane = 4 r + block for full-resolution row r = 2 j + sy of the 16-row band): the two lanes
-// of a block row exchange their pixel pairs (lane ^ 4) to form the 2x2 means, and both carry
-// that block row through the transforms.
+// block row of a full-resolution channel (lane = 8 j + block: rows load and store coalesced),
+// or 4 blocks of 4:2:0 chroma, whose 16-row band it loads row-contiguously and hands to the
+// transform layout (lane = 8 j + 4 sy + block, both sy lanes carrying component row j) through
+// a wave-private LDS tile (lane_unit_sub22).
 
-// Value of the same register in lane ^ X (X = 4: ds_swizzle bitmask mode; X = 8: DPP row_ror:8).
-template <int X>
-__device__ __forceinline__ float lane_xor(float v)
+// Value of the same register in lane ^ 8 (DPP row_ror:8).
+__device__ __forceinline__ float lane_xor8(float v)
 {
-    const int b = __float_as_int(v);
-    static_assert(X == 4 || X == 8, "lane_xor: 4 or 8");
-    if constexpr (X == 8) return __int_as_float(__builtin_amdgcn_update_dpp(0, b, 0x128, 0xF, 0xF, false));
-    else return __int_as_float(__builtin_amdgcn_ds_swizzle(b, 0x101F));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x128, 0xF, 0xF, false));
 }
 
 // Lane with block row j (lane bits 3-5) holds a[k] = M[j][k]; afterwards a[k] = M[k][j].
@@ -1935,7 +1931,7 @@ __device__ __forceinline__ void transpose8(float a[8], int j)
 #pragma unroll
     for (int k = 0; k < 8; k += 2) {
         const float lo_v = a[k], hi_v = a[k + 1];
-        const float recv = lane_xor<8>(hi ? lo_v : hi_v);
+        const float recv = lane_xor8(hi ? lo_v : hi_v);
         a[k] = hi ? recv : lo_v;
         a[k + 1] = hi ? hi_v : recv;
     }
@@ -2010,8 +2006,10 @@ __device__ __forceinline__ void lane_store_prob(const ChanDev &ch, int by, int b
 // within their own lane group, and store nothing.
 
 // A unit's y, g and coefficient loads.  Full resolution: one block row of 8 pixels (y[0..1],
-// g[0..1]); 4:2:0 chroma: 16 pixels of one full-resolution row (y[0..3], g[0..3]).  An invalid
-// lane loads from offset 0.
+// g[0..1]).  4:2:0 chroma: the unit's band of 16 grid rows by 64 columns row-contiguously, load
+// q covering rows 4q .. 4q + 3 (lane 16 h + s: pixels 4s .. 4s + 3 of row 4q + h, in y[q] and
+// g[q]), so that each load instruction reads four whole 256-byte row segments.  A lane of a
+// block past the channel's width loads from offset 0.
 struct LaneLoad {
     float4 y[4], g[4];
     int4 raw;
@@ -2019,20 +2017,30 @@ struct LaneLoad {
 template <int MODE>
 __device__ __forceinline__ void lane_load(const IterArgs &a, int c, int by, int bx, int r, bool valid, LaneLoad &L)
 {
-    constexpr bool SUB = MODE == PM_SUB22;
-    constexpr int N4 = SUB ? 4 : 2;
     const ChanDev &ch = a.ch[c];
-    const int j = SUB ? r >> 1 : r;
     const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
-    const size_t p = !valid ? 0 : (SUB ? (size_t)(by * 16 + r) * a.W + bx * 16 : (size_t)(by * 8 + r) * a.W + bx * 8);
-    const float4 *yp = reinterpret_cast<const float4 *>(a.y[c] + p);
-    const float4 *gp = reinterpret_cast<const float4 *>(a.g[c] + p);
+    if constexpr (MODE == PM_SUB22) {
+        const int lane = threadIdx.x & 63, h = lane >> 4, s = lane & 15;
+        const int bx0 = bx - (lane & 3);  // the unit's first block
+        const bool lvalid = bx0 + (s >> 2) < (int)ch.bw;
 #pragma unroll
-    for (int q4 = 0; q4 < N4; q4++) {
-        L.y[q4] = yp[q4];
-        L.g[q4] = gp[q4];
+        for (int q = 0; q < 4; q++) {
+            const size_t p = lvalid ? (size_t)(by * 16 + 4 * q + h) * a.W + (size_t)bx0 * 16 + 4 * s : 0;
+            L.y[q] = *reinterpret_cast<const float4 *>(a.y[c] + p);
+            L.g[q] = *reinterpret_cast<const float4 *>(a.g[c] + p);
+        }
+        L.raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + (r >> 1) * 8);
+    } else {
+        const size_t p = valid ? (size_t)(by * 8 + r) * a.W + bx * 8 : 0;
+        const float4 *yp = reinterpret_cast<const float4 *>(a.y[c] + p);
+        const float4 *gp = reinterpret_cast<const float4 *>(a.g[c] + p);
+#pragma unroll
+        for (int q4 = 0; q4 < 2; q4++) {
+            L.y[q4] = yp[q4];
+            L.g[q4] = gp[q4];
+        }
+        L.raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + r * 8);
     }
-    L.raw = *reinterpret_cast<const int4 *>(ch.coef + blk * 64 + j * 8);
 }
 
 // Full-resolution channel: 8 lanes per block, block row j = lane >> 3.
@@ -2064,20 +2072,30 @@ __device__ __forceinline__ void lane_unit_full(const IterArgs &a, const NormShar
     }
 }
 
-// 4:2:0 chroma: 16 lanes per block, full-resolution row r = lane >> 2 of the 16-row band; block
-// row j = r / 2 is held twice (rows 2j and 2j + 1, lanes 4 apart), and the upper row's lane
-// stores the block's outputs.
+// Value of v in lane ^ 16: v_permlane16_swap of v with itself gives the odd 16-lane rows the
+// even rows' values (first result) and the even rows the odd rows' (second).
+__device__ __forceinline__ float lane_xor16(float v, bool odd_row)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(odd_row ? r[0] : r[1]);
+}
+
+// 4:2:0 chroma.  The band arrives row-contiguously (lane_load: lane 16 h + s holds pixels
+// 4s .. 4s + 3 of rows 4q + h).  The rows of a 2x2 box, 4q + h and 4q + (h ^ 1), sit in lanes 16
+// apart.  A wave-private LDS tile of the unit's 8 x 32 component pixels carries the box means
+// into the transform layout (lane 8 j + 4 sy + b: component row j of block b, held by both sy
+// lanes) and the projected means back; a wave's LDS accesses complete in program order, so it
+// needs no barrier.  Every value goes through the same operations as in k_project_fused.
 __device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, const NormShared &ns, int c, int by, int bx, int r,
-                                                bool valid, const LaneLoad &L)
+                                                bool valid, const LaneLoad &L, float (*tile)[36])
 {
     const ChanDev &ch = a.ch[c];
     const float norm = ns.norm[c];
     const double rn = ns.rnorm[c];
-    const int j = r >> 1, sy = r & 1;
+    const int lane = threadIdx.x & 63, h = lane >> 4, s = lane & 15, b = lane & 3;
+    const int j = r >> 1, sy = r & 1, bx0 = bx - b;
+    const bool lvalid = bx0 + (s >> 2) < (int)ch.bw, lower = (h & 1) != 0;
     const size_t blk = valid ? (size_t)by * ch.bw + bx : 0;
-    const size_t p = valid ? (size_t)(by * 16 + r) * a.W + bx * 16 : 0;
-    const float4 *yv = L.y, *gv = L.g;
-    const int4 raw = L.raw;
     float q8[8];
     double rq8[8];
 #pragma unroll
@@ -2085,41 +2103,57 @@ __device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, const NormSha
         q8[k] = ch.quant[j * 8 + k];
         rq8[k] = ch.rqq[j * 8 + k];
     }
-    float yd[16];
+    float yd[16];  // yd[4q + t]: pixel 4s + t of band row 4q + h
 #pragma unroll
-    for (int q4 = 0; q4 < 4; q4++) lane_descend(norm, rn, a.step, yv[q4], gv[q4], &yd[4 * q4]);
+    for (int q = 0; q < 4; q++) lane_descend(norm, rn, a.step, L.y[q], L.g[q], &yd[4 * q]);
     // Box means in the reference's order (compute_projection.rs:21-31): the upper row's pair,
-    // then the lower row's; both lanes of a block row form the same sums.
-    float v[8], pr[8];
+    // then the lower row's; both lanes of a row pair form the same sums.
+    float mq[8];  // mq[2q + e]: component row 2q + (h >> 1), column 2s + e
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float t0 = lane_xor<4>(yd[2 * k]), t1 = lane_xor<4>(yd[2 * k + 1]);
-        const float u0 = sy ? t0 : yd[2 * k], u1 = sy ? t1 : yd[2 * k + 1];
-        const float l0 = sy ? yd[2 * k] : t0, l1 = sy ? yd[2 * k + 1] : t1;
-        float m = 0.0f;
-        m += u0;
-        m += u1;
-        m += l0;
-        m += l1;
-        m /= 4.0f;
-        v[k] = m;
-        yd[2 * k] -= m;
-        yd[2 * k + 1] -= m;
-    }
-    lane_project_block(v, j, ch, blk, raw, q8, rq8, valid, sy == 0, a.write_cos != 0, pr);
-    if (valid) {
-        float o[16];
+    for (int q = 0; q < 4; q++) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            o[2 * k] = yd[2 * k] + v[k];
-            o[2 * k + 1] = yd[2 * k + 1] + v[k];
+        for (int e = 0; e < 2; e++) {
+            const float a0 = yd[4 * q + 2 * e], a1 = yd[4 * q + 2 * e + 1];
+            const float b0 = lane_xor16(a0, lower), b1 = lane_xor16(a1, lower);
+            const float u0 = lower ? b0 : a0, u1 = lower ? b1 : a1;
+            const float l0 = lower ? a0 : b0, l1 = lower ? a1 : b1;
+            float m = 0.0f;
+            m += u0;
+            m += u1;
+            m += l0;
+            m += l1;
+            m /= 4.0f;
+            mq[2 * q + e] = m;
+            yd[4 * q + 2 * e] -= m;
+            yd[4 * q + 2 * e + 1] -= m;
         }
-        float *dst = a.xnew[c] + p;
-#pragma unroll
-        for (int q4 = 0; q4 < 4; q4++)
-            *reinterpret_cast<float4 *>(dst + 4 * q4) = make_float4(o[4 * q4], o[4 * q4 + 1], o[4 * q4 + 2], o[4 * q4 + 3]);
-        if (sy == 0) lane_store_prob(ch, by, bx, j, pr);
     }
+    // Both lanes of a pair store the same means (a lane-dependent choice of registers would
+    // send the array to memory).
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        *reinterpret_cast<float2 *>(&tile[2 * q + (h >> 1)][2 * s]) = make_float2(mq[2 * q], mq[2 * q + 1]);
+    __builtin_amdgcn_wave_barrier();
+    const float4 t0 = *reinterpret_cast<const float4 *>(&tile[j][8 * b]);
+    const float4 t1 = *reinterpret_cast<const float4 *>(&tile[j][8 * b + 4]);
+    __builtin_amdgcn_wave_barrier();
+    float v[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w}, pr[8];
+    lane_project_block(v, j, ch, blk, L.raw, q8, rq8, valid, sy == 0, a.write_cos != 0, pr);
+    // The projected means back; the two sy lanes of a block row hold and store the same values.
+    *reinterpret_cast<float4 *>(&tile[j][8 * b]) = make_float4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<float4 *>(&tile[j][8 * b + 4]) = make_float4(v[4], v[5], v[6], v[7]);
+    __builtin_amdgcn_wave_barrier();
+    if (lvalid) {
+        float *dst = a.xnew[c] + (size_t)(by * 16 + h) * a.W + (size_t)bx0 * 16 + 4 * s;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float2 pm = *reinterpret_cast<const float2 *>(&tile[2 * q + (h >> 1)][2 * s]);
+            *reinterpret_cast<float4 *>(dst + (size_t)4 * q * a.W) =
+                make_float4(yd[4 * q] + pm.x, yd[4 * q + 1] + pm.x, yd[4 * q + 2] + pm.y, yd[4 * q + 3] + pm.y);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (valid && sy == 0) lane_store_prob(ch, by, bx, j, pr);
 }
 
 // Blocks per wave: 8 for full-resolution channels, 4 for 4:2:0 chroma.
@@ -2143,6 +2177,7 @@ template <int C, int M0, int M1, int M2>
 __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t u0, uint32_t u1, uint32_t u2)
 {
     __shared__ NormShared ns;
+    __shared__ float ctile[NT / 64][8][36];  // 4:2:0 chroma units: each wave's component tile
     const int lane = threadIdx.x & 63;
     projection_norms<C>(a, ns);  // before any wave leaves: it has a barrier
     // A grid of fewer than NXCD workgroups leaves XCDs without one: then every workgroup takes
@@ -2184,7 +2219,7 @@ __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t 
             const bool valid = bx < (int)ch.bw;
             LaneLoad L;
             lane_load<MODE>(a, c, by, bx, lane / BPW, valid, L);
-            if constexpr (MODE == PM_SUB22) lane_unit_sub22(a, ns, c, by, bx, lane / BPW, valid, L);
+            if constexpr (MODE == PM_SUB22) lane_unit_sub22(a, ns, c, by, bx, lane / BPW, valid, L, ctile[threadIdx.x >> 6]);
             else lane_unit_full(a, ns, c, by, bx, lane / BPW, valid, L);
         };
         if (unit < u0) run(IC<M0>{}, IC<0>{}, unit);
