@@ -2072,6 +2072,16 @@ __device__ __forceinline__ void lane_unit_full(const IterArgs &a, const NormShar
     }
 }
 
+// Orders a wave's LDS accesses across its lanes: the wavefront-scope release/acquire pair makes
+// the stores before it visible to the loads after it, and wave_barrier keeps the compiler from
+// moving memory accesses across it.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Value of v in lane ^ 16: v_permlane16_swap of v with itself gives the odd 16-lane rows the
 // even rows' values (first result) and the even rows the odd rows' (second).
 __device__ __forceinline__ float lane_xor16(float v, bool odd_row)
@@ -2084,8 +2094,8 @@ __device__ __forceinline__ float lane_xor16(float v, bool odd_row)
 // 4s .. 4s + 3 of rows 4q + h).  The rows of a 2x2 box, 4q + h and 4q + (h ^ 1), sit in lanes 16
 // apart.  A wave-private LDS tile of the unit's 8 x 32 component pixels carries the box means
 // into the transform layout (lane 8 j + 4 sy + b: component row j of block b, held by both sy
-// lanes) and the projected means back; a wave's LDS accesses complete in program order, so it
-// needs no barrier.  Every value goes through the same operations as in k_project_fused.
+// lanes) and the projected means back, with wave_sync() between the stores and the loads (no
+// workgroup barrier).  Every value goes through the same operations as in k_project_fused.
 __device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, const NormShared &ns, int c, int by, int bx, int r,
                                                 bool valid, const LaneLoad &L, float (*tile)[36])
 {
@@ -2133,16 +2143,16 @@ __device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, const NormSha
 #pragma unroll
     for (int q = 0; q < 4; q++)
         *reinterpret_cast<float2 *>(&tile[2 * q + (h >> 1)][2 * s]) = make_float2(mq[2 * q], mq[2 * q + 1]);
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     const float4 t0 = *reinterpret_cast<const float4 *>(&tile[j][8 * b]);
     const float4 t1 = *reinterpret_cast<const float4 *>(&tile[j][8 * b + 4]);
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     float v[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w}, pr[8];
     lane_project_block(v, j, ch, blk, L.raw, q8, rq8, valid, sy == 0, a.write_cos != 0, pr);
     // The projected means back; the two sy lanes of a block row hold and store the same values.
     *reinterpret_cast<float4 *>(&tile[j][8 * b]) = make_float4(v[0], v[1], v[2], v[3]);
     *reinterpret_cast<float4 *>(&tile[j][8 * b + 4]) = make_float4(v[4], v[5], v[6], v[7]);
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     if (lvalid) {
         float *dst = a.xnew[c] + (size_t)(by * 16 + h) * a.W + (size_t)bx0 * 16 + 4 * s;
 #pragma unroll
@@ -2152,7 +2162,7 @@ __device__ __forceinline__ void lane_unit_sub22(const IterArgs &a, const NormSha
                 make_float4(yd[4 * q] + pm.x, yd[4 * q + 1] + pm.x, yd[4 * q + 2] + pm.y, yd[4 * q + 3] + pm.y);
         }
     }
-    __builtin_amdgcn_wave_barrier();
+    wave_sync();
     if (valid && sy == 0) lane_store_prob(ch, by, bx, j, pr);
 }
 
