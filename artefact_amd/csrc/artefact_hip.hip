@@ -1768,10 +1768,16 @@ __constant__ float kDct8[64] = {
     0x1.8f8b84p-4f, -0x1.1c73b4p-2f, 0x1.a9b662p-2f, -0x1.f6297cp-2f, 0x1.f6297cp-2f, -0x1.a9b662p-2f, 0x1.1c73b4p-2f, -0x1.8f8b84p-4f,
 };
 
-template <int C>
+// SUB: channels 1 and 2 are 4:2:0 chroma (C = 3).  A workgroup's TW x TH tile then holds one
+// super-tile of each chroma component (4 component blocks side by side, 32 x 8 component
+// pixels): every wave forms the 2x2 box means of its pixels into an LDS tile, waves 0 and 1 run
+// the transforms of Cb and Cr on it, and every wave adds the projected means back.
+template <int C, bool SUB = false>
 __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
 {
+    static_assert(!SUB || C == 3, "k_project_mfma: 4:2:0 chroma needs three channels");
     __shared__ NormShared ns;
+    __shared__ __align__(16) float ctile[SUB ? 2 : 1][8][36];  // SUB: each chroma component's 8 x 32 tile
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int W = a.W, H = a.H;
     // The wave's four 8x8 blocks sit side by side in one 8-row band: super-tile blocks (0,0),
@@ -1818,11 +1824,11 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
     for (int c = 0; c < C; c++) {
         yl[c] = *reinterpret_cast<const float4 *>(a.y[c] + pix);
         gl[c] = *reinterpret_cast<const float4 *>(a.g[c] + pix);
-        cl[c] = *reinterpret_cast<const int2 *>(a.ch[c].coef + coef_index(c));
+        if (!SUB || c == 0) cl[c] = *reinterpret_cast<const int2 *>(a.ch[c].coef + coef_index(c));
     }
     projection_norms<C>(a, ns);
 #pragma unroll
-    for (int c = 0; c < C; c++) {
+    for (int c = 0; c < (SUB ? 1 : C); c++) {
         const ChanDev &ch = a.ch[c];
         const float norm = ns.norm[c];
         const double rn = ns.rnorm[c];
@@ -1882,6 +1888,109 @@ __global__ __launch_bounds__(NT) void k_project_mfma(const IterArgs a)
                 *reinterpret_cast<float4 *>(ch.prob + (size_t)py * ch.rw + px) =
                     make_float4(ap * pn[0], ap * pn[1], ap * pn[2], ap * pn[3]);
             }
+        }
+    }
+    if constexpr (SUB) {
+        // This lane's pixels (row py, columns px .. px + 3) are component row py / 2, columns
+        // px / 2 and px / 2 + 1: in the workgroup's component tile, row 4 (wave / 2) + (r & 7) / 2
+        // and columns 16 (wave & 1) + 8 (r / 8) + cq / 2 + e.  The other row of each 2x2 box is
+        // in lane ^ 1 (r ^ 1).
+        const int trow = 4 * (wave >> 1) + ((r & 7) >> 1), tcol = 16 * (wave & 1) + 8 * (r >> 3) + (cq >> 1);
+        const bool lower = (r & 1) != 0;
+        float res[2][4];
+#pragma unroll
+        for (int cc = 0; cc < 2; cc++) {
+            const int c = cc + 1;
+            const float norm = ns.norm[c];
+            const double rn = ns.rnorm[c];
+            const float ys[4] = {yl[c].x, yl[c].y, yl[c].z, yl[c].w};
+            const float gs[4] = {gl[c].x, gl[c].y, gl[c].z, gl[c].w};
+            float v[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                float y = ys[t];
+                if (norm != 0.0f) y -= step * qdiv(gs[t], rn);  // compute_step.rs:61-65
+                v[t] = in ? y : 0.0f;
+            }
+            // Box means in the reference's order (compute_projection.rs:21-31): the upper row's
+            // pair, then the lower row's; both lanes of the pair form the same sums.
+            float m[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float a0 = v[2 * e], a1 = v[2 * e + 1];
+                const float b0 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a0), 0xB1, 0xF, 0xF, false));
+                const float b1 = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(a1), 0xB1, 0xF, 0xF, false));
+                const float u0 = lower ? b0 : a0, u1 = lower ? b1 : a1;
+                const float l0 = lower ? a0 : b0, l1 = lower ? a1 : b1;
+                float mm = 0.0f;
+                mm += u0;
+                mm += u1;
+                mm += l0;
+                mm += l1;
+                mm /= 4.0f;
+                m[e] = mm;
+                res[cc][2 * e] = v[2 * e] - mm;
+                res[cc][2 * e + 1] = v[2 * e + 1] - mm;
+            }
+            *reinterpret_cast<float2 *>(&ctile[cc][trow][tcol]) = make_float2(m[0], m[1]);
+        }
+        __syncthreads();
+        if (wave < 2) {
+            // Wave cc projects chroma component cc + 1's four blocks: lane (r, cq) holds
+            // component row cy, columns cx .. cx + 3, in the same A layout as the luma tiles.
+            const int cc = wave, c = cc + 1;
+            const ChanDev &ch = a.ch[c];
+            const int cy = (int)ty * (TH / 2) + (r & 7), cx = (int)tx * (TW / 2) + 16 * (r >> 3) + cq;
+            const bool cin = cx < (int)ch.rw && cy < (int)ch.rh;
+            const uint32_t ci = cin ? ((uint32_t)(cy >> 3) * ch.bw + (uint32_t)(cx >> 3)) * 64u + (uint32_t)kq : 0u;
+            const int2 craw = *reinterpret_cast<const int2 *>(ch.coef + ci);
+            const float4 tv = *reinterpret_cast<const float4 *>(&ctile[cc][r & 7][16 * (r >> 3) + cq]);
+            const f32x4 xv = {tv.x, tv.y, tv.z, tv.w};
+            const f32x4 yv = mfma_forward_stage(mfma_forward_stage(xv, fwd, lane), fwd, lane);
+            const bool dc_lane = (lane & 40) == lane;
+            const float4 q4 = *reinterpret_cast<const float4 *>(ch.quant + kq);
+            const float qs[4] = {q4.x, q4.y, q4.z, q4.w};
+            const int cw[2] = {craw.x, craw.y};
+            f32x4 ov, dv;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const float q = qs[t];
+                const float cf = (float)(short)((uint32_t)cw[t >> 1] >> (16 * (t & 1)));
+                float o = clamp_ref(yv[t], (cf - 0.5f) * q, (cf + 0.5f) * q);
+                float d = qdiv(o - cf * q, ch.rqq[kq + t]);
+                ov[t] = cin ? o : 0.0f;
+                dv[t] = cin ? d : 0.0f;
+            }
+            if (cin && a.write_cos)
+                *reinterpret_cast<float4 *>(ch.cos + ci) = make_float4(ov[0], ov[1], ov[2], ov[3]);
+            const float odc = __shfl(ov[0], lane & 40, 64) * 0.125f;
+            f32x4 oac = ov;
+            if (dc_lane) oac[0] = 0.0f;
+            f32x4 xn = mfma_stage(mfma_stage(oac, inv), inv);
+#pragma unroll
+            for (int t = 0; t < 4; t++) xn[t] += odc;
+            if (ch.alpha_p != 0.0f) {
+                const float ddc = __shfl(dv[0], lane & 40, 64) * 0.125f;
+                f32x4 dac = dv;
+                if (dc_lane) dac[0] = 0.0f;
+                f32x4 pn = mfma_stage(mfma_stage(dac, inv), inv);
+                const float ap = ch.alpha_p;
+                if (cin)
+                    *reinterpret_cast<float4 *>(ch.prob + (size_t)cy * ch.rw + cx) =
+                        make_float4(ap * (pn[0] + ddc), ap * (pn[1] + ddc), ap * (pn[2] + ddc), ap * (pn[3] + ddc));
+            }
+            // The projected means replace the box means in the tile (this lane's own four).
+            *reinterpret_cast<float4 *>(&ctile[cc][r & 7][16 * (r >> 3) + cq]) = make_float4(xn[0], xn[1], xn[2], xn[3]);
+        }
+        __syncthreads();
+        // compute_projection.rs:108-121: each pixel's difference from its box mean on top of
+        // the projected mean.
+#pragma unroll
+        for (int cc = 0; cc < 2; cc++) {
+            const float2 pm = *reinterpret_cast<const float2 *>(&ctile[cc][trow][tcol]);
+            if (in)
+                *reinterpret_cast<float4 *>(a.xnew[cc + 1] + pix) =
+                    make_float4(res[cc][0] + pm.x, res[cc][1] + pm.x, res[cc][2] + pm.y, res[cc][3] + pm.y);
         }
     }
 }
@@ -2187,7 +2296,7 @@ template <int C, int M0, int M1, int M2>
 __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t u0, uint32_t u1, uint32_t u2)
 {
     __shared__ NormShared ns;
-    __shared__ float ctile[NT / 64][8][36];  // 4:2:0 chroma units: each wave's component tile
+    __shared__ __align__(16) float ctile[NT / 64][8][36];  // 4:2:0 chroma units: each wave's component tile
     const int lane = threadIdx.x & 63;
     projection_norms<C>(a, ns);  // before any wave leaves: it has a barrier
     // A grid of fewer than NXCD workgroups leaves XCDs without one: then every workgroup takes
@@ -2396,6 +2505,7 @@ void launch_project(const artefact_hip_plan *pl, const IterArgs &a, hipStream_t 
     if (pl->precision == ARTEFACT_PRECISION_BF16X3) {
         if (pl->C == 1) hipLaunchKernelGGL((k_project_mfma<1>), pl->grid_p, dim3(NT), 0, s, a);
         else if (pl->C == 2) hipLaunchKernelGGL((k_project_mfma<2>), pl->grid_p, dim3(NT), 0, s, a);
+        else if (pl->ch[1].mode == PM_SUB22) hipLaunchKernelGGL((k_project_mfma<3, true>), pl->grid_p, dim3(NT), 0, s, a);
         else hipLaunchKernelGGL((k_project_mfma<3>), pl->grid_p, dim3(NT), 0, s, a);
         return;
     }
@@ -3346,9 +3456,15 @@ int artefact_hip_plan_set_precision(artefact_hip_plan *pl, uint32_t precision)
     if (precision != ARTEFACT_PRECISION_F32 && precision != ARTEFACT_PRECISION_BF16X3 &&
         precision != ARTEFACT_PRECISION_F32_FAST)
         return ARTEFACT_E_ARGUMENT;
-    if (precision == ARTEFACT_PRECISION_BF16X3)
-        for (uint32_t c = 0; c < pl->C; c++)
-            if (pl->ch[c].mode != PM_FULL) return ARTEFACT_E_GEOMETRY;
+    if (precision == ARTEFACT_PRECISION_BF16X3) {
+        // Every channel at full resolution, or 4:2:0 colour (full-resolution luma, both chroma
+        // components 2 x 2 subsampled).
+        const bool sub420 = pl->C == 3 && pl->ch[0].mode == PM_FULL && pl->ch[1].mode == PM_SUB22 &&
+                            pl->ch[2].mode == PM_SUB22;
+        if (!sub420)
+            for (uint32_t c = 0; c < pl->C; c++)
+                if (pl->ch[c].mode != PM_FULL) return ARTEFACT_E_GEOMETRY;
+    }
     if (pl->precision != precision) {
         DeviceGuard guard(pl->device);
         if (pl->stream) (void)hipStreamSynchronize(pl->stream);

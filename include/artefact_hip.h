@@ -135,7 +135,7 @@ void artefact_hip_plan_destroy(artefact_hip_plan *plan);
  * gradient sweep with v_sqrt_f32 / v_rcp_f32 and (alpha / n) * term in place of the correctly
  * rounded (alpha * term) / n, and a plain f64 sum for the norm; projection unchanged; within a
  * few ulp per term, not bit-identical).  BF16X3 needs every channel at full grid resolution
- * (4:4:4 or one channel), else ARTEFACT_E_GEOMETRY.  No reference counterpart. */
+ * (4:4:4 or one channel) or 4:2:0 colour, else ARTEFACT_E_GEOMETRY.  No reference counterpart. */
 int artefact_hip_plan_set_precision(artefact_hip_plan *plan, uint32_t precision);
 
 /* Autotuner result: the k_gradient strip height (output rows per wave) and the projection

@@ -1,4 +1,5 @@
-"""Precision form: the projection's DCT/IDCTs on BF16 MFMA (hi/lo split, f32 accumulation).
+"""Precision form: the projection's DCT/IDCTs on BF16 MFMA (hi/lo split, f32 accumulation), for
+full-resolution channels and 4:2:0 colour.
 
 Not bit-identical to the reference by design; measured against the FP64 oracle loop
 (oracle.compute_f64), next to the plain f32 path.  Measured on MI355X (4:4:4 96x64, 0..255
@@ -11,6 +12,7 @@ import pytest
 import artefact_amd
 import oracle
 from artefact_amd import synth
+from checks import requant_deviation
 
 pytestmark = pytest.mark.gpu
 
@@ -27,7 +29,7 @@ def run(wl, iters, precision):
 
 
 @pytest.mark.parametrize("name,iters", [("tgv_444_96x64", 10), ("tv_luma_64x64", 10), ("tgv_444_96x64", 100),
-                                        ("tv_luma_64x64", 50)])
+                                        ("tv_luma_64x64", 50), ("tgv_420_96x64", 10), ("tgv_420_208x144", 30)])
 def test_bf16x3_error_vs_fp64_oracle(name, iters):
     """Per channel RMS error <= 3 x the f32 path's + 0.01, relative objective error <= 1e-4.
     Max-abs and the 99.9th percentile are printed, not bounded: wherever a joint norm is near 0,
@@ -64,8 +66,20 @@ def test_bf16x3_requantises():
         assert dev.max() <= 0.5 + 0.05, dev.max()
 
 
-def test_bf16x3_needs_full_resolution_channels():
-    wl = synth.workload("tgv_420_96x64", seed=1)
+@pytest.mark.parametrize("name", ["tgv_420_96x64", "tgv_420_208x144"])
+def test_bf16x3_420_requantises(name):
+    """4:2:0 colour: every luma block and every chroma block of the 2x2 box means re-quantises to
+    its input coefficients within 1/2 plus the transform error (0.05)."""
+    wl = synth.workload(name, seed=2)
+    W, H = wl["max_w"], wl["max_h"]
+    print(f"case precision requantise {name}", flush=True)
+    out, _ = run(wl, 20, "bf16x3")
+    assert requant_deviation(out, wl["coefs"], W, H) <= 0.5 + 0.05
+
+
+def test_bf16x3_needs_full_resolution_or_420_channels():
+    """Layouts other than full resolution and 4:2:0 colour are refused."""
+    wl = synth.workload("tgv_422_96x64", seed=1)
     with artefact_amd.Plan(wl["coefs"], 0.3, wl["pweight"], 2, wl["max_w"], wl["max_h"]) as plan:
         with pytest.raises(artefact_amd.ArtefactError) as e:
             plan.set_precision("bf16x3")
