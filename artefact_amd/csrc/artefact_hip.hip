@@ -2342,7 +2342,19 @@ __global__ __launch_bounds__(NT) void k_project_lane(const IterArgs a, uint32_t 
             else lane_unit_full(a, ns, c, by, bx, lane / BPW, valid, L);
         };
         if (unit < u0) run(IC<M0>{}, IC<0>{}, unit);
-        else if (C > 1 && unit < u1) run(IC<M1>{}, IC<1>{}, unit - u0);
+        else if constexpr (C > 2 && M1 == M2 && M1 == PM_SUB22) {
+            // Both chroma components through one copy of the code, the channel a (wave-uniform)
+            // index: a smaller kernel for the instruction cache.
+            const int c = unit < u1 ? 1 : 2;
+            const uint32_t local = unit - (c == 1 ? u0 : u1);
+            const ChanDev &ch = a.ch[c];
+            const uint32_t per_row = (ch.bw + 3) / 4;
+            const int by = (int)(local / per_row), bx = (int)(local % per_row) * 4 + lane % 4;
+            const bool valid = bx < (int)ch.bw;
+            LaneLoad L;
+            lane_load<PM_SUB22>(a, c, by, bx, lane / 4, valid, L);
+            lane_unit_sub22(a, ns, c, by, bx, lane / 4, valid, L, ctile[threadIdx.x >> 6]);
+        } else if (C > 1 && unit < u1) run(IC<M1>{}, IC<1>{}, unit - u0);
         else if (C > 2) run(IC<M2>{}, IC<2>{}, unit - u1);
     }
 }
