@@ -3486,6 +3486,19 @@ int artefact_hip_plan_set_precision(artefact_hip_plan *pl, uint32_t precision)
     return ARTEFACT_OK;
 }
 
+int artefact_hip_plan_set_momentum_offset(artefact_hip_plan *pl, uint32_t k)
+{
+    if (!pl) return ARTEFACT_E_ARGUMENT;
+    if (k > (1u << 20)) return ARTEFACT_E_ARGUMENT;
+    float term = 1.0f;
+    for (uint32_t i = 0; i < k; i++) (void)fista_factor(term);
+    for (uint32_t i = 0; i < pl->iterations; i++) pl->factors[i] = fista_factor(term);
+    DeviceGuard guard(pl->device);
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    drop_graph(pl);
+    return ARTEFACT_OK;
+}
+
 int artefact_hip_plan_set_step_iterations(artefact_hip_plan *pl, uint32_t n)
 {
     if (!pl) return ARTEFACT_E_ARGUMENT;

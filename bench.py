@@ -328,8 +328,8 @@ def measure_batch(imgs, steps=5, warmup=2, tune=True):
 
 
 def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
-    """CNN prediction -> projection -> K plain iterations (the step of the 100-iteration schedule)
-    against the plain 100-iteration solve: times, objectives, the fallback decision, and the
+    """CNN prediction -> projection -> K plain iterations (the step of the 250-iteration schedule,
+    the FISTA recurrence started 40 steps in) against the plain 100-iteration solve: times, objectives, the fallback decision, and the
     planes (decoded, plain 100, warm, plain K)."""
     import artefact_amd
 
@@ -367,7 +367,8 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
     planes["decoded"] = plain.download()
     plain.close()
     warm = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
-    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, 100), "step")
+    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, WARM15[0]), "step")
+    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_momentum_offset(warm._h, WARM15[1]), "momentum")
     res[f"cnn_plus_{K}_ms"] = round(timed(warm, warm_start), 4)
     warm_start(warm)
     warm.iterate()
@@ -401,50 +402,55 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
     return res, planes
 
 
-def warm_time_to_objective(net, coefs, W, H, f_plain, plain_ms, timed, warm_start, schedules=(150, 250, 400),
-                          tolerance=1e-3):
+# CNN + 15: the step of the 250-iteration schedule, the FISTA recurrence started 40 steps in.
+WARM15 = (250, 40)
+# Refinement settings the time-to-objective search tries: (step schedule S, momentum offset T).
+WARM_SETTINGS = ((250, 0), (250, 40), (400, 40), (400, 50), (400, 60), (400, 70), (400, 80), (600, 60), (600, 80))
+
+
+def warm_time_to_objective(net, coefs, W, H, f_plain, plain_ms, timed, warm_start, settings=WARM_SETTINGS,
+                           tolerance=1e-3):
     """The fewest plain iterations K after the CNN start whose objective reaches plain 100's
-    within `tolerance` (the target's 0.1 %), for refinement steps of an S-iteration schedule:
-    bisection over K in 15..100 for each S (the objective falls with K), then the fastest (K, S)
-    timed like the other solves."""
+    within `tolerance` (the target's 0.1 %), for refinement steps of an S-iteration schedule and
+    the FISTA recurrence started T steps in: for each (S, T) a scan of K = 15, 20, .., 100, then
+    one by one below the first K that reaches (with a late start the objective need not fall
+    monotonically with K), and the fastest (K, S, T) timed like the other solves."""
     import artefact_amd
 
     target = f_plain * (1.0 + tolerance)
 
     pw = [0.001] * 3
 
-    def objective(K, S):
+    def objective(K, S, T):
         with artefact_amd.Plan(coefs, 0.3, pw, K, W, H) as p:
             artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_iterations(p._h, S), "step")
+            artefact_amd._lib.check(p._lib.artefact_hip_plan_set_momentum_offset(p._h, T), "momentum")
             warm_start(p)
             p.iterate()
             return p.objective()[0]
 
     best = None
-    for S in schedules:
-        lo, hi = 15, 100
-        if objective(hi, S) > target:
+    for S, T in settings:
+        hit = next((K for K in range(15, 101, 5) if objective(K, S, T) <= target), None)
+        if hit is None:
             continue
-        while hi - lo > 1:
-            mid = (lo + hi) // 2
-            if objective(mid, S) <= target:
-                hi = mid
-            else:
-                lo = mid
-        if best is None or hi < best[0]:
-            best = (hi, S)
+        hit = next((K for K in range(max(15, hit - 4), hit) if objective(K, S, T) <= target), hit)
+        if best is None or hit < best[0]:
+            best = (hit, S, T)
     criterion = f"objective <= plain 100's x {1.0 + tolerance:g}"
     if best is None:
         return {"reached": False, "criterion": criterion,
-                "searched": "K in 15..100, steps of the %s-iteration schedules" % list(schedules)}
-    K, S = best
+                "searched": "K in 15..100 for (step schedule, momentum offset) in %s" % list(settings)}
+    K, S, T = best
     with artefact_amd.Plan(coefs, 0.3, pw, K, W, H) as p:
         artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_iterations(p._h, S), "step")
+        artefact_amd._lib.check(p._lib.artefact_hip_plan_set_momentum_offset(p._h, T), "momentum")
         ms = timed(p, warm_start)
         warm_start(p)
         p.iterate()
         f = p.objective()[0]
-    return {"reached": True, "criterion": criterion, "cnn_plus_K": K, "step_iterations": S, "ms": round(ms, 4),
+    return {"reached": True, "criterion": criterion, "cnn_plus_K": K, "step_iterations": S, "momentum_offset": T,
+            "ms": round(ms, 4),
             "objective_rel_to_plain": float(f"{(f - f_plain) / f_plain:.4g}"),
             "fraction_of_plain_time": round(ms / plain_ms, 4)}
 
@@ -540,8 +546,9 @@ def other_configs(args, pre):
         w["fp64_converged_reference"] = ("tests/golden/config_results.json (oracle.compute_f64, 300 iterations; "
                                          "subsample every 8th row and column)")
     out["warm_start_420_2048_q50"] = {"workload": "learned warm start: TGV 4:2:0 2048x1536 IJG q50, CNN + 15 plain "
-                                                  "iterations (step of the 100-iteration schedule, fast f32 "
-                                                  "gradient sweep) vs plain 100 (exact)", **w}
+                                                  "iterations (step of the 250-iteration schedule, FISTA recurrence "
+                                                  "started 40 steps in, fast f32 gradient sweep) vs plain 100 "
+                                                  "(exact)", **w}
     print(f"bench: warm start {w['cnn_plus_15_ms']:.3f} ms vs plain {w['plain_100_ms']:.3f} ms", file=sys.stderr,
           flush=True)
     return out
@@ -678,7 +685,8 @@ def bench_warmstart(args) -> int:
         artefact_amd._lib.check(p._lib.artefact_hip_plan_warm_init(p._h, net._h), "artefact_hip_plan_warm_init")
 
     warm = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
-    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, 100), "step")
+    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, WARM15[0]), "step")
+    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_momentum_offset(warm._h, WARM15[1]), "momentum")
     t_warm = timed(warm, warm_start)
     warm_start(warm)
     warm.iterate()

@@ -258,6 +258,12 @@ int artefact_hip_plan_warm_init(artefact_hip_plan *plan, artefact_cnn *net);
  * A warm-started refinement of K iterations can take the step of a longer schedule. */
 int artefact_hip_plan_set_step_iterations(artefact_hip_plan *plan, uint32_t n);
 
+/* Starts the plan's FISTA recurrence (pipeline_scalar/mod.rs:50-52) k steps in: the loop's
+ * first iteration takes the extrapolation factor of the recurrence's step k + 1, as if k
+ * iterations had already run (0 restores the reference's start).  For a refinement from a start
+ * already near the result, such as the warm start's; k <= 2^20.  No reference counterpart. */
+int artefact_hip_plan_set_momentum_offset(artefact_hip_plan *plan, uint32_t k);
+
 /* ---- JPEG input: Jpeg::from (backend/artefact-lib/jpeg/zune.rs:5-62) on the host ----
  * Decodes baseline/extended sequential and progressive Huffman JPEGs (8-bit, 1-4 components,
  * sampling factors 1 or 2) to the reference's Coefficient layout: per component the plane

@@ -72,3 +72,29 @@ def test_fallback_returns_the_plain_result():
         for a, b in zip(res.planes, plain):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     assert res.objective <= f_plain
+
+
+def test_momentum_offset():
+    """artefact_hip_plan_set_momentum_offset: offset 0 gives the reference's loop bit for bit (also
+    after another offset was set), an offset leaves the first iteration alone (x = x_prev there,
+    so the extrapolation factor multiplies zero) and changes the later ones, and offsets above
+    2^20 are refused."""
+    wl = synth.workload("tgv_420_96x64", seed=4)
+    W, H = wl["max_w"], wl["max_h"]
+    print("case momentum offset tgv_420_96x64", flush=True)
+
+    def solve(iters, offsets):
+        with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], iters, W, H, mode=wl["mode"]) as p:
+            for k in offsets:
+                artefact_amd._lib.check(p._lib.artefact_hip_plan_set_momentum_offset(p._h, k), "momentum")
+            p.init()
+            p.iterate()
+            return p.download()
+
+    base = solve(6, [])
+    assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(base, solve(6, [40, 0])))
+    one = solve(1, []), solve(1, [40])
+    assert all(np.array_equal(a.view(np.uint32), b.view(np.uint32)) for a, b in zip(*one))
+    assert any(not np.array_equal(a, b) for a, b in zip(base, solve(6, [40])))
+    with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], 2, W, H, mode=wl["mode"]) as p:
+        assert p._lib.artefact_hip_plan_set_momentum_offset(p._h, (1 << 20) + 1) == -1
