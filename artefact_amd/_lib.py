@@ -161,6 +161,7 @@ SIGNATURES = [
     ("artefact_hip_plan_warm_init", ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     ("artefact_hip_plan_set_step_iterations", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     ("artefact_hip_plan_set_momentum_offset", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    ("artefact_hip_plan_set_step_schedule", ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32]),
     ("artefact_jpeg_decode", ctypes.c_int,
      [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]),
     ("artefact_jpeg_nchannel", ctypes.c_uint32, [ctypes.c_void_p]),

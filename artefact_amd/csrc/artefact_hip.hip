@@ -2380,6 +2380,7 @@ struct artefact_hip_plan {
     float qtab[3][64] = {};                       // quantisation tables per channel, f32 (warm-start features)
     float alpha_tv = 0.0f, alpha_tgv = 0.0f, step = 0.0f;
     std::vector<float> factors;
+    std::vector<float> steps;  // per-iteration steps (artefact_hip_plan_set_step_schedule); empty: `step`
     ChanDev ch[3];
     void *arena = nullptr;               // the one device allocation every buffer below lives in
     float *buf[2] = {nullptr, nullptr};  // each C*N: x and x_prev, swapped every iteration
@@ -2448,6 +2449,12 @@ void free_plan(artefact_hip_plan *pl)
     if (pl->arena) (void)hipFree(pl->arena);
     if (pl->stream) (void)hipStreamDestroy(pl->stream);
     delete pl;
+}
+
+// The step of iteration `it`: the plan's constant step, or its decaying schedule.
+float step_at(const artefact_hip_plan *pl, uint32_t it)
+{
+    return pl->steps.empty() ? pl->step : pl->steps[it];
 }
 
 IterArgs make_args(const artefact_hip_plan *pl, int cur, float factor, bool write_cos)
@@ -2564,7 +2571,8 @@ int launch_iterations(artefact_hip_plan *pl, hipStream_t s)
     hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, s, pl->acc);
     int cur = pl->cur;
     for (uint32_t it = 0; it < pl->iterations; it++) {
-        const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == pl->iterations);
+        IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == pl->iterations);
+        a.step = step_at(pl, it);
         launch_gradient(pl, a, s);
         launch_project(pl, a, s);
         cur = 1 - cur;
@@ -2836,7 +2844,8 @@ float time_loop(artefact_hip_plan *pl, uint32_t nt)
             hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, pl->stream, pl->acc);
             int cur = 0;
             for (uint32_t it = 0; it < nt; it++) {
-                const IterArgs a = make_args(pl, cur, pl->factors[it], false);
+                IterArgs a = make_args(pl, cur, pl->factors[it], false);
+                a.step = step_at(pl, it);
                 launch_gradient(pl, a, pl->stream);
                 launch_project(pl, a, pl->stream);
                 cur = 1 - cur;
@@ -3268,7 +3277,8 @@ int artefact_hip_plan_profile(artefact_hip_plan *pl, float *avg_ms)
         hipLaunchKernelGGL(k_zero_acc, dim3(1), dim3(NT), 0, pl->stream, pl->acc);
         ok = hipEventRecord(ev[0], pl->stream) == hipSuccess;
         for (uint32_t it = 0; ok && it < n; it++) {
-            const IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == n);
+            IterArgs a = make_args(pl, cur, pl->factors[it], it + 1 == n);
+            a.step = step_at(pl, it);
             launch_gradient(pl, a, pl->stream);
             ok = ok && hipEventRecord(ev[2 * it + 1], pl->stream) == hipSuccess;
             launch_project(pl, a, pl->stream);
@@ -3499,12 +3509,29 @@ int artefact_hip_plan_set_momentum_offset(artefact_hip_plan *pl, uint32_t k)
     return ARTEFACT_OK;
 }
 
+int artefact_hip_plan_set_step_schedule(artefact_hip_plan *pl, uint32_t first, uint32_t last)
+{
+    const int rc = artefact_hip_plan_set_step_iterations(pl, last);  // the constant step; clears a schedule
+    if (rc != ARTEFACT_OK || first == 0) return rc;
+    const float radius = sqrtf((float)pl->N) / 2.0f;
+    const float s1 = radius / sqrtf(1.0f + (float)first), s2 = pl->step;
+    try {
+        pl->steps.resize(pl->iterations);
+    } catch (...) {
+        return ARTEFACT_E_HIP;
+    }
+    for (uint32_t i = 0; i < pl->iterations; i++)
+        pl->steps[i] = pl->iterations > 1 ? s1 * powf(s2 / s1, (float)i / (float)(pl->iterations - 1)) : s2;
+    return ARTEFACT_OK;
+}
+
 int artefact_hip_plan_set_step_iterations(artefact_hip_plan *pl, uint32_t n)
 {
     if (!pl) return ARTEFACT_E_ARGUMENT;
     const float radius = sqrtf((float)pl->N) / 2.0f;
     const uint32_t k = n ? n : pl->iterations;
     pl->step = radius / sqrtf(1.0f + (float)k);  // pipeline_scalar/mod.rs:45,64-74 with k in place of iterations
+    pl->steps.clear();
     DeviceGuard guard(pl->device);
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     drop_graph(pl);

@@ -241,24 +241,26 @@ class WarmResult:
 
 def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective: float | None = None,
                weight: float = 0.3, pweight=(0.001, 0.001, 0.001), plain_iterations: int = 100,
-               step_iterations: int | None = None, momentum_offset: int = 0,
+               step_iterations: int | None = None, step_from: int = 0, momentum_offset: int = 0,
                precision: str = "f32_fast") -> WarmResult:
     """CNN prediction -> projection -> ``iterations`` plain iterations with the step of the
-    ``step_iterations`` schedule (default: ``plain_iterations``) and the FISTA recurrence started
+    ``step_iterations`` schedule (default: ``plain_iterations``; with ``step_from`` the step decays
+    from the ``step_from`` schedule's to it over the refinement) and the FISTA recurrence started
     ``momentum_offset`` steps in (default 0: as the reference starts it); if the result's
     objective is higher than the plain ``plain_iterations`` result's, the plain result is
     returned.
 
     The prediction is not bit-exact to anything (the CNN runs in bf16), so the refinement takes
     the fast f32 gradient sweep by default (``precision``); the plain result is the exact loop.
-    Two settings measured at 2048x1536 4:2:0 q50 against plain 100 (6.4 ms): 15 iterations with
-    the step of 250 and the recurrence started 40 steps in take 22 % of its time and end 4.1 %
-    above its objective (the fallback is taken); 40 iterations with the step of 400 and the
-    recurrence started 60 steps in reach its objective within 0.1 % in 43 % of its time."""
+    Two settings measured at 2048x1536 4:2:0 q50 against plain 100 (6.3 ms): 15 iterations with
+    the step decaying from the 100- to the 600-iteration schedule's and the recurrence started 40
+    steps in take 22 % of its time and end 3.6 % above its objective (the fallback is taken); 28
+    iterations with the step decaying from 50 to 600 and the recurrence started 40 steps in reach
+    its objective within 0.1 % in 32.5 % of its time."""
     with Plan(coefs, weight, list(pweight), iterations, W, H) as plan:
         plan.set_precision(precision)
-        _lib.check(plan._lib.artefact_hip_plan_set_step_iterations(
-            plan._h, plain_iterations if step_iterations is None else step_iterations), "step")
+        _lib.check(plan._lib.artefact_hip_plan_set_step_schedule(
+            plan._h, step_from, plain_iterations if step_iterations is None else step_iterations), "step")
         _lib.check(plan._lib.artefact_hip_plan_set_momentum_offset(plan._h, momentum_offset), "momentum")
         _lib.check(plan._lib.artefact_hip_plan_warm_init(plan._h, net._h), "artefact_hip_plan_warm_init")
         plan.iterate()

@@ -328,8 +328,9 @@ def measure_batch(imgs, steps=5, warmup=2, tune=True):
 
 
 def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
-    """CNN prediction -> projection -> K plain iterations (the step of the 250-iteration schedule,
-    the FISTA recurrence started 40 steps in) against the plain 100-iteration solve: times, objectives, the fallback decision, and the
+    """CNN prediction -> projection -> K plain iterations (WARM15: the step decaying from the
+    100- to the 600-iteration schedule's, the FISTA recurrence started 40 steps in) against the
+    plain 100-iteration solve: times, objectives, the fallback decision, and the
     planes (decoded, plain 100, warm, plain K)."""
     import artefact_amd
 
@@ -367,8 +368,7 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
     planes["decoded"] = plain.download()
     plain.close()
     warm = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
-    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, WARM15[0]), "step")
-    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_momentum_offset(warm._h, WARM15[1]), "momentum")
+    set_refinement(warm, *WARM15)
     res[f"cnn_plus_{K}_ms"] = round(timed(warm, warm_start), 4)
     warm_start(warm)
     warm.iterate()
@@ -402,54 +402,65 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
     return res, planes
 
 
-# CNN + 15: the step of the 250-iteration schedule, the FISTA recurrence started 40 steps in.
-WARM15 = (250, 40)
-# Refinement settings the time-to-objective search tries: (step schedule S, momentum offset T).
-WARM_SETTINGS = ((250, 0), (250, 40), (400, 40), (400, 50), (400, 60), (400, 70), (400, 80), (600, 60), (600, 80))
+# Warm refinement settings (S1, S2, T): the step decays from the S1-iteration schedule's to the
+# S2-iteration schedule's over the refinement (S1 = 0: the constant step of S2), and the FISTA
+# recurrence starts T steps in.  CNN + 15 takes WARM15; the time-to-objective search tries
+# WARM_SETTINGS.
+WARM15 = (100, 600, 40)
+WARM_SETTINGS = ((0, 250, 0), (0, 400, 50), (0, 400, 60), (100, 600, 40), (50, 600, 40), (50, 600, 20),
+                 (30, 600, 40), (50, 1000, 40), (100, 1000, 60))
+
+
+def set_refinement(p, S1, S2, T):
+    import artefact_amd
+
+    artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_schedule(p._h, S1, S2), "step schedule")
+    artefact_amd._lib.check(p._lib.artefact_hip_plan_set_momentum_offset(p._h, T), "momentum")
 
 
 def warm_time_to_objective(net, coefs, W, H, f_plain, plain_ms, timed, warm_start, settings=WARM_SETTINGS,
                            tolerance=1e-3):
     """The fewest plain iterations K after the CNN start whose objective reaches plain 100's
-    within `tolerance` (the target's 0.1 %), for refinement steps of an S-iteration schedule and
-    the FISTA recurrence started T steps in: for each (S, T) a scan of K = 15, 20, .., 100, then
-    one by one below the first K that reaches (with a late start the objective need not fall
-    monotonically with K), and the fastest (K, S, T) timed like the other solves."""
+    within `tolerance` (the target's 0.1 %), for refinement steps decaying from an S1- to an
+    S2-iteration schedule's and
+    the FISTA recurrence started T steps in: for each (S1, S2, T) of `settings` (see WARM15) a scan
+    of K = 15, 20, .., 100, then one by one below the first K that reaches (with a late start the
+    objective need not fall monotonically with K), and the fastest setting timed like the other
+    solves."""
     import artefact_amd
 
     target = f_plain * (1.0 + tolerance)
 
     pw = [0.001] * 3
 
-    def objective(K, S, T):
+    def objective(K, S1, S2, T):
         with artefact_amd.Plan(coefs, 0.3, pw, K, W, H) as p:
-            artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_iterations(p._h, S), "step")
-            artefact_amd._lib.check(p._lib.artefact_hip_plan_set_momentum_offset(p._h, T), "momentum")
+            set_refinement(p, S1, S2, T)
             warm_start(p)
             p.iterate()
             return p.objective()[0]
 
     best = None
-    for S, T in settings:
-        hit = next((K for K in range(15, 101, 5) if objective(K, S, T) <= target), None)
+    for S1, S2, T in settings:
+        hit = next((K for K in range(15, 101, 5) if objective(K, S1, S2, T) <= target), None)
         if hit is None:
             continue
-        hit = next((K for K in range(max(15, hit - 4), hit) if objective(K, S, T) <= target), hit)
+        hit = next((K for K in range(max(15, hit - 4), hit) if objective(K, S1, S2, T) <= target), hit)
         if best is None or hit < best[0]:
-            best = (hit, S, T)
+            best = (hit, S1, S2, T)
     criterion = f"objective <= plain 100's x {1.0 + tolerance:g}"
     if best is None:
         return {"reached": False, "criterion": criterion,
                 "searched": "K in 15..100 for (step schedule, momentum offset) in %s" % list(settings)}
-    K, S, T = best
+    K, S1, S2, T = best
     with artefact_amd.Plan(coefs, 0.3, pw, K, W, H) as p:
-        artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_iterations(p._h, S), "step")
-        artefact_amd._lib.check(p._lib.artefact_hip_plan_set_momentum_offset(p._h, T), "momentum")
+        set_refinement(p, S1, S2, T)
         ms = timed(p, warm_start)
         warm_start(p)
         p.iterate()
         f = p.objective()[0]
-    return {"reached": True, "criterion": criterion, "cnn_plus_K": K, "step_iterations": S, "momentum_offset": T,
+    return {"reached": True, "criterion": criterion, "cnn_plus_K": K, "step_from_iterations": S1,
+            "step_iterations": S2, "momentum_offset": T,
             "ms": round(ms, 4),
             "objective_rel_to_plain": float(f"{(f - f_plain) / f_plain:.4g}"),
             "fraction_of_plain_time": round(ms / plain_ms, 4)}
@@ -546,9 +557,9 @@ def other_configs(args, pre):
         w["fp64_converged_reference"] = ("tests/golden/config_results.json (oracle.compute_f64, 300 iterations; "
                                          "subsample every 8th row and column)")
     out["warm_start_420_2048_q50"] = {"workload": "learned warm start: TGV 4:2:0 2048x1536 IJG q50, CNN + 15 plain "
-                                                  "iterations (step of the 250-iteration schedule, FISTA recurrence "
-                                                  "started 40 steps in, fast f32 gradient sweep) vs plain 100 "
-                                                  "(exact)", **w}
+                                                  "iterations (step decaying from the 100- to the 600-iteration "
+                                                  "schedule's, FISTA recurrence started 40 steps in, fast f32 "
+                                                  "gradient sweep) vs plain 100 (exact)", **w}
     print(f"bench: warm start {w['cnn_plus_15_ms']:.3f} ms vs plain {w['plain_100_ms']:.3f} ms", file=sys.stderr,
           flush=True)
     return out
@@ -685,8 +696,7 @@ def bench_warmstart(args) -> int:
         artefact_amd._lib.check(p._lib.artefact_hip_plan_warm_init(p._h, net._h), "artefact_hip_plan_warm_init")
 
     warm = artefact_amd.Plan(coefs, 0.3, pw, K, W, H)
-    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_step_iterations(warm._h, WARM15[0]), "step")
-    artefact_amd._lib.check(warm._lib.artefact_hip_plan_set_momentum_offset(warm._h, WARM15[1]), "momentum")
+    set_refinement(warm, *WARM15)
     t_warm = timed(warm, warm_start)
     warm_start(warm)
     warm.iterate()

@@ -264,6 +264,14 @@ int artefact_hip_plan_set_step_iterations(artefact_hip_plan *plan, uint32_t n);
  * already near the result, such as the warm start's; k <= 2^20.  No reference counterpart. */
 int artefact_hip_plan_set_momentum_offset(artefact_hip_plan *plan, uint32_t k);
 
+/* Lets the step decay over the loop: iteration i of n takes s_first^(1 - i/(n-1)) *
+ * s_last^(i/(n-1)), where s_m = sqrt(N)/2/sqrt(1 + m) is the step of an m-iteration schedule
+ * (pipeline_scalar/mod.rs:45,64-74; m = 0: the plan's iteration count).  first = 0 gives the
+ * constant step of artefact_hip_plan_set_step_iterations(plan, last), which also clears a
+ * schedule.  For a refinement from a start already near the result, such as the warm start's.
+ * No reference counterpart. */
+int artefact_hip_plan_set_step_schedule(artefact_hip_plan *plan, uint32_t first, uint32_t last);
+
 /* ---- JPEG input: Jpeg::from (backend/artefact-lib/jpeg/zune.rs:5-62) on the host ----
  * Decodes baseline/extended sequential and progressive Huffman JPEGs (8-bit, 1-4 components,
  * sampling factors 1 or 2) to the reference's Coefficient layout: per component the plane

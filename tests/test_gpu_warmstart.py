@@ -98,3 +98,26 @@ def test_momentum_offset():
     assert any(not np.array_equal(a, b) for a, b in zip(base, solve(6, [40])))
     with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], 2, W, H, mode=wl["mode"]) as p:
         assert p._lib.artefact_hip_plan_set_momentum_offset(p._h, (1 << 20) + 1) == -1
+
+
+def test_step_schedule():
+    """artefact_hip_plan_set_step_schedule: first = 0 is set_step_iterations(last) bit for bit, as
+    is a schedule from a step to the same step; a decaying schedule changes the result."""
+    wl = synth.workload("tgv_420_96x64", seed=5)
+    W, H = wl["max_w"], wl["max_h"]
+    print("case step schedule tgv_420_96x64", flush=True)
+
+    def solve(setup):
+        with artefact_amd.Plan(wl["coefs"], wl["weight"], wl["pweight"], 6, W, H, mode=wl["mode"]) as p:
+            setup(p)
+            p.init()
+            p.iterate()
+            return p.download()
+
+    def same(a, b):
+        return all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(a, b))
+
+    const = solve(lambda p: artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_iterations(p._h, 400), "s"))
+    assert same(const, solve(lambda p: artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_schedule(p._h, 0, 400), "s")))
+    assert same(const, solve(lambda p: artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_schedule(p._h, 400, 400), "s")))
+    assert not same(const, solve(lambda p: artefact_amd._lib.check(p._lib.artefact_hip_plan_set_step_schedule(p._h, 50, 400), "s")))
