@@ -407,8 +407,8 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
 # recurrence starts T steps in.  CNN + 15 takes WARM15; the time-to-objective search tries
 # WARM_SETTINGS.
 WARM15 = (100, 600, 40)
-WARM_SETTINGS = ((0, 250, 0), (0, 400, 50), (0, 400, 60), (100, 600, 40), (50, 600, 40), (50, 600, 20),
-                 (30, 600, 40), (50, 1000, 40), (100, 1000, 60))
+WARM_SETTINGS = ((0, 400, 60), (50, 600, 40), (15, 600, 20), (6, 1500, 20), (3, 600, 20), (3, 1500, 10),
+                 (3, 600, 10))
 
 
 def set_refinement(p, S1, S2, T):

@@ -206,22 +206,22 @@ def test_warm_start_full_size():
 
 
 def test_warm_start_reaches_plain_objective():
-    """2048x1536 4:2:0 q50: 30 iterations after the CNN start (the step decaying from the 50- to
-    the 600-iteration schedule's, the FISTA recurrence started 40 steps in) end within 0.1 % of
-    plain 100's objective, in 30 % of the plain loop's iterations; 70 iterations (step of 250,
+    """2048x1536 4:2:0 q50: 25 iterations after the CNN start (the step decaying from the 3- to
+    the 1500-iteration schedule's, the FISTA recurrence started 10 steps in) end within 0.1 % of
+    plain 100's objective, in 25 % of the plain loop's iterations; 70 iterations (step of 250,
     recurrence from t = 1) reach it without the fallback.  Every block still re-quantises."""
     coefs = synth.encode(synth.synth_rgb(1536, 2048, 0), subsampling="420", quality=50)
     W, H = coefs[0].rounded_px_w, coefs[0].rounded_px_h
-    print("case warm start 2048x1536 q50, K=30 (steps 50 -> 600, offset 40) and K=70 (step of 250)", flush=True)
+    print("case warm start 2048x1536 q50, K=25 (steps 3 -> 1500, offset 10) and K=70 (step of 250)", flush=True)
     _, f_plain = solve({"coefs": coefs, "weight": 0.3, "pweight": [0.001] * 3, "iterations": 100, "max_w": W,
                         "max_h": H, "mode": "tgv"})
     net = warmstart.WarmStartNet.load()
-    fast = warmstart.warm_solve(coefs, W, H, net, iterations=30, plain_objective=f_plain, plain_iterations=100,
-                                step_iterations=600, step_from=50, momentum_offset=40)
+    fast = warmstart.warm_solve(coefs, W, H, net, iterations=25, plain_objective=f_plain, plain_iterations=100,
+                                step_iterations=1500, step_from=3, momentum_offset=10)
     res = warmstart.warm_solve(coefs, W, H, net, iterations=70, plain_objective=f_plain, plain_iterations=100,
                                step_iterations=250)
     net.close()
-    print(f"K=30: {(fast.warm_objective - f_plain) / f_plain:+.3e}; K=70: {(res.warm_objective - f_plain) / f_plain:+.3e}",
+    print(f"K=25: {(fast.warm_objective - f_plain) / f_plain:+.3e}; K=70: {(res.warm_objective - f_plain) / f_plain:+.3e}",
           flush=True)
     assert fast.warm_objective <= f_plain * 1.001
     assert requant_deviation(fast.warm_planes, coefs, W, H) <= 0.5 + 1e-3
