@@ -253,8 +253,8 @@ def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective:
     The prediction is not bit-exact to anything (the CNN runs in bf16), so the refinement takes
     the fast f32 gradient sweep by default (``precision``); the plain result is the exact loop.
     Two settings measured at 2048x1536 4:2:0 q50 against plain 100 (6.3 ms): 15 iterations with
-    the step decaying from the 100- to the 600-iteration schedule's and the recurrence started 40
-    steps in take 22 % of its time and end 3.6 % above its objective (the fallback is taken); 25
+    the step decaying from the 3- to the 600-iteration schedule's and the recurrence started 10
+    steps in take 22 % of its time and end 2.2 % above its objective (the fallback is taken); 25
     iterations with the step decaying from 6 to 1500 and the recurrence started 20 steps in reach
     its objective within 0.1 % in 30 % of its time."""
     with Plan(coefs, weight, list(pweight), iterations, W, H) as plan:

@@ -329,7 +329,7 @@ def measure_batch(imgs, steps=5, warmup=2, tune=True):
 
 def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
     """CNN prediction -> projection -> K plain iterations (WARM15: the step decaying from the
-    100- to the 600-iteration schedule's, the FISTA recurrence started 40 steps in) against the
+    3- to the 600-iteration schedule's, the FISTA recurrence started 10 steps in) against the
     plain 100-iteration solve: times, objectives, the fallback decision, and the
     planes (decoded, plain 100, warm, plain K)."""
     import artefact_amd
@@ -406,7 +406,7 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
 # S2-iteration schedule's over the refinement (S1 = 0: the constant step of S2), and the FISTA
 # recurrence starts T steps in.  CNN + 15 takes WARM15; the time-to-objective search tries
 # WARM_SETTINGS.
-WARM15 = (100, 600, 40)
+WARM15 = (3, 600, 10)
 WARM_SETTINGS = ((0, 400, 60), (50, 600, 40), (15, 600, 20), (6, 1500, 20), (3, 600, 20), (3, 1500, 10),
                  (3, 600, 10))
 
@@ -557,8 +557,8 @@ def other_configs(args, pre):
         w["fp64_converged_reference"] = ("tests/golden/config_results.json (oracle.compute_f64, 300 iterations; "
                                          "subsample every 8th row and column)")
     out["warm_start_420_2048_q50"] = {"workload": "learned warm start: TGV 4:2:0 2048x1536 IJG q50, CNN + 15 plain "
-                                                  "iterations (step decaying from the 100- to the 600-iteration "
-                                                  "schedule's, FISTA recurrence started 40 steps in, fast f32 "
+                                                  "iterations (step decaying from the 3- to the 600-iteration "
+                                                  "schedule's, FISTA recurrence started 10 steps in, fast f32 "
                                                   "gradient sweep) vs plain 100 (exact)", **w}
     print(f"bench: warm start {w['cnn_plus_15_ms']:.3f} ms vs plain {w['plain_100_ms']:.3f} ms", file=sys.stderr,
           flush=True)
