@@ -254,9 +254,9 @@ def warm_solve(coefs, W, H, net: WarmStartNet, iterations: int, plain_objective:
     the fast f32 gradient sweep by default (``precision``); the plain result is the exact loop.
     Two settings measured at 2048x1536 4:2:0 q50 against plain 100 (6.3 ms): 15 iterations with
     the step decaying from the 3- to the 600-iteration schedule's and the recurrence started 10
-    steps in take 22 % of its time and end 2.2 % above its objective (the fallback is taken); 25
-    iterations with the step decaying from 6 to 1500 and the recurrence started 20 steps in reach
-    its objective within 0.1 % in 30 % of its time."""
+    steps in take 22 % of its time and end 2.2 % above its objective (the fallback is taken); 23
+    iterations with the step decaying from 2 to 1500 and the recurrence started 10 steps in reach
+    its objective within 0.1 % in 28 % of its time."""
     with Plan(coefs, weight, list(pweight), iterations, W, H) as plan:
         plan.set_precision(precision)
         _lib.check(plan._lib.artefact_hip_plan_set_step_schedule(

@@ -408,7 +408,7 @@ def measure_warm(net, coefs, W, H, K=15, steps=5, warmup=2):
 # WARM_SETTINGS.
 WARM15 = (3, 600, 10)
 WARM_SETTINGS = ((0, 400, 60), (50, 600, 40), (15, 600, 20), (6, 1500, 20), (3, 600, 20), (3, 1500, 10),
-                 (3, 600, 10))
+                 (2, 1500, 10), (1, 1500, 10))
 
 
 def set_refinement(p, S1, S2, T):
